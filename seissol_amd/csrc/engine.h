@@ -1,0 +1,58 @@
+// Internal engine state and kernel entry points shared by the HIP sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/seissol_amd.h"
+
+namespace sdg {
+
+void setError(const std::string& msg);
+int hipFail(hipError_t err, const char* what);
+#define SDG_HIP_CHECK(call)                                  \
+  do {                                                       \
+    hipError_t err_ = (call);                                \
+    if (err_ != hipSuccess) return ::sdg::hipFail(err_, #call); \
+  } while (0)
+
+constexpr int kMaxOrder = 7;
+constexpr int kMinOrder = 2;
+
+// Offsets (in reals) of the device copies of the global matrices.  All
+// matrices are stored transposed relative to the generator output so that
+// consecutive output rows are consecutive in memory:
+//   ck  [3][B][B]    ck[j][l][k]   = -kDivMT_j[k][l]
+//   vol [3][B][B]    vol[j][l][k]  =  kDivM_j[k][l]
+//   fmrt[4][B][B2]   fmrt[f][l][m] =  fMrT_f[m][l]
+//   rt  [4][B][B2]   rt[j][l][m]   =  rT_j[m][l]
+//   fp  [3][B2][B2]  fp[h][n][m]   =  fP_h[m][n]
+//   rdm [4*B2][B]    rdm[f*B2+m][k] = rDivM_f[k][m]
+struct DeviceMatrixLayout {
+  int64_t ck, vol, fmrt, rt, fp, rdm, total;
+};
+DeviceMatrixLayout deviceMatrixLayout(int order);
+
+struct EngineState {
+  int order = 0;
+  int arithmetic = SDG_FP64;
+  void* matrices = nullptr;  // device, real type of the engine
+  DeviceMatrixLayout layout{};
+};
+
+}  // namespace sdg
+
+struct sdg_engine {
+  sdg::EngineState s;
+};
+
+namespace sdg {
+// kernels_valu.hip
+int launchAder(const EngineState& e, const double* coeffs, const sdg_layer* layer, hipStream_t stream);
+int launchAderLocal(const EngineState& e, const double* coeffs, const sdg_layer* layer, hipStream_t stream);
+int launchLocal(const EngineState& e, const sdg_layer* layer, hipStream_t stream);
+int launchNeighbor(const EngineState& e, const sdg_layer* layer, hipStream_t stream);
+int launchTaylor(const EngineState& e, const double* coeffs, const void* derivatives, void* out, int64_t n,
+                 hipStream_t stream);
+}  // namespace sdg

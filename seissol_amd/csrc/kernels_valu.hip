@@ -1,0 +1,675 @@
+// Element-update kernels on the vector ALUs (FP64 drop-in and FP32).
+//
+// Work decomposition: a wave owns EPW elements; each element is handled by
+// TPE consecutive lanes and lane t of an element owns the basis rows
+// t, t + TPE, ..., so every small GEMM Y(M x 9) += A(M x K) X(K x 9) is a
+// register-blocked R x 9 outer-product accumulation per lane.  The global
+// matrix A is read transposed (consecutive rows = consecutive addresses, all
+// elements of a wave read the same lines); the per-element operand X lives in
+// LDS in [row][quantity] order so one row of 9 quantities is a broadcast read.
+//
+// The CK recursion and the volume integral are evaluated as
+//   sum_j (A_j X) star_j
+// i.e. three accumulators sets followed by the sparse 9x9 star product
+// (24 non-zeros, pattern of codegen/matrices/star.xml), which keeps the star
+// matrices out of the inner loop.  Face terms use the factored forms of the
+// reference CPU kernels (codegen/kernels/aderdg.py:354-431):
+//   local:     rDivM_f ((fMrT_f I) AplusT_f)
+//   neighbour: rDivM_f ((fP_h (rT_j I_nb)) AminusT_f)
+// with the four faces' rDivM products fused into one K = 4*B2 contraction.
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+
+namespace sdg {
+namespace {
+
+constexpr int basis3(int o) { return o * (o + 1) * (o + 2) / 6; }
+constexpr int basis2(int o) { return o * (o + 1) / 2; }
+
+template <int O>
+struct Shape {
+  static constexpr int B = basis3(O);
+  static constexpr int B2 = basis2(O);
+  static constexpr int R = B <= 10 ? 1 : (B <= 56 ? 2 : 3);
+  static constexpr int TPE = (B + R - 1) / R;
+  static constexpr int EPW = 64 / TPE;
+  static constexpr int WAVES = 4;
+  static constexpr int EPB = EPW * WAVES;
+  static constexpr int THREADS = 64 * WAVES;
+  static constexpr int QS = 9 * B;       // reals per DOF block
+  static constexpr int FS = 4 * B2 * 9;  // reals of the four face operands
+  // device matrix offsets (must match deviceMatrixLayout)
+  static constexpr int64_t ck = 0;
+  static constexpr int64_t vol = ck + 3LL * B * B;
+  static constexpr int64_t fmrt = vol + 3LL * B * B;
+  static constexpr int64_t rt = fmrt + 4LL * B * B2;
+  static constexpr int64_t fp = rt + 4LL * B * B2;
+  static constexpr int64_t rdm = fp + 3LL * B2 * B2;
+  static constexpr int64_t total = rdm + 4LL * B2 * B;
+};
+
+// Sparsity of the elastic star matrix: for output quantity p the non-zero
+// input quantities q (codegen/matrices/star.xml, 0-based).
+__device__ constexpr int kStarQ[9][3] = {{6, 7, 8}, {6, 7, 8}, {6, 7, 8}, {6, 7, -1}, {7, 8, -1},
+                                         {6, 8, -1}, {0, 3, 5}, {1, 3, 4}, {2, 4, 5}};
+constexpr int kStarNnz = 24;  // per direction
+
+template <int O, typename real>
+struct Coeffs {
+  real c[O];
+};
+
+// Gather the 24 non-zeros of the three star matrices of the elements of the
+// block into LDS (compact order: p-major, pattern order).
+template <typename real>
+__device__ inline void loadStars(real* sStar, const real* __restrict__ star, int64_t e0, int64_t n, int epb) {
+  for (int idx = threadIdx.x; idx < epb * 3 * kStarNnz; idx += blockDim.x) {
+    const int s = idx / (3 * kStarNnz);
+    const int w = idx % (3 * kStarNnz);
+    const int j = w / kStarNnz;
+    int z = w % kStarNnz;
+    int p = 0, k = 0;
+    // position z -> (p, k)
+    for (p = 0; p < 9; ++p) {
+      const int cnt = (p >= 3 && p <= 5) ? 2 : 3;
+      if (z < cnt) { k = z; break; }
+      z -= cnt;
+    }
+    const int q = kStarQ[p][k];
+    const int64_t e = e0 + s;
+    sStar[idx] = (e < n) ? star[e * 243 + 81 * j + q + 9 * p] : real(0);
+  }
+}
+
+// y[r][p] = sum_j sum_q acc[j][r][q] * star_j(q, p) with the compact stars of one element
+template <int R, typename real>
+__device__ inline void applyStars(const real (&acc)[3][R][9], const real* sS, real (&y)[R][9]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int p = 0; p < 9; ++p) y[r][p] = real(0);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    int z = 0;
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+      const int cnt = (p >= 3 && p <= 5) ? 2 : 3;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (k < cnt) {
+          const real sv = sS[j * kStarNnz + z + k];
+          const int q = kStarQ[p][k];
+#pragma unroll
+          for (int r = 0; r < R; ++r) y[r][p] += acc[j][r][q] * sv;
+        }
+      }
+      z += cnt;
+    }
+  }
+}
+
+// acc[j][r][p] += sum_{l<K} AT_j[l][row_r] * X[l][p], j < J, rows < M
+template <int J, int R, typename real>
+__device__ inline void gemmRows(real (&acc)[J][R][9], const real* __restrict__ AT, int64_t strideJ, int ldAT,
+                                const real* X, int K, int t, int tpe, int M) {
+  for (int l = 0; l < K; ++l) {
+    real x[9];
+#pragma unroll
+    for (int p = 0; p < 9; ++p) x[p] = X[l * 9 + p];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int row = t + r * tpe;
+        const real a = row < M ? AT[j * strideJ + static_cast<int64_t>(l) * ldAT + row] : real(0);
+#pragma unroll
+        for (int p = 0; p < 9; ++p) acc[j][r][p] += a * x[p];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ader (+ optional volume and local flux): Time.cpp:65-140 and Local.cpp:124-160
+// ---------------------------------------------------------------------------
+template <typename real, int O, bool WithLocal>
+__global__ __launch_bounds__(256) void kAder(int64_t n, const real* __restrict__ mats, Coeffs<O, real> cf,
+                                             real* __restrict__ dofs, real* __restrict__ integ,
+                                             real* __restrict__ deriv, int64_t dsize,
+                                             const real* __restrict__ star, const real* __restrict__ aplus,
+                                             const uint8_t* __restrict__ faceTypes) {
+  using S = Shape<O>;
+  constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
+  __shared__ real sA[S::EPB * S::QS];
+  __shared__ real sB[S::EPB * (S::FS > S::QS ? S::FS : S::QS)];
+  __shared__ real sStar[S::EPB * 3 * kStarNnz];
+  constexpr int ldB = (S::FS > S::QS ? S::FS : S::QS);
+
+  const int64_t e0 = static_cast<int64_t>(blockIdx.x) * S::EPB;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int slot = wave * S::EPW + lane / TPE;
+  const int t = lane % TPE;
+  const bool laneOk = lane < S::EPW * TPE;
+  const int64_t e = e0 + slot;
+  const bool active = laneOk && e < n;
+
+  // stage Q as [row][quantity]
+  for (int idx = threadIdx.x; idx < S::EPB * S::QS; idx += blockDim.x) {
+    const int s = idx / S::QS, w = idx % S::QS;
+    const int p = w / B, l = w % B;
+    const int64_t ee = e0 + s;
+    sA[s * S::QS + l * 9 + p] = (ee < n) ? dofs[ee * S::QS + w] : real(0);
+  }
+  loadStars(sStar, star, e0, n, S::EPB);
+  __syncthreads();
+
+  const int sl = laneOk ? slot : 0;
+  real* xa = sA + sl * S::QS;
+  real* xb = sB + sl * ldB;
+  const real* sS = sStar + sl * 3 * kStarNnz;
+
+  real qreg[R][9], iacc[R][9];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int row = t + r * TPE;
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+      qreg[r][p] = row < B ? xa[row * 9 + p] : real(0);
+      iacc[r][p] = cf.c[0] * qreg[r][p];
+    }
+  }
+  if (deriv != nullptr && active) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = t + r * TPE;
+      if (row < B)
+        for (int p = 0; p < 9; ++p) deriv[e * dsize + p * B + row] = qreg[r][p];
+    }
+  }
+
+  int64_t doff = 9 * B;
+  int nIn = B;
+#pragma unroll 1
+  for (int d = 1; d < O; ++d) {
+    const int m = basis3(O - d);
+    const real* src = (d & 1) ? xa : xb;
+    real* dst = (d & 1) ? xb : xa;
+    real acc[3][R][9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int p = 0; p < 9; ++p) acc[j][r][p] = real(0);
+    gemmRows<3, R>(acc, mats + S::ck, static_cast<int64_t>(B) * B, B, src, nIn, t, TPE, m);
+    real dq[R][9];
+    applyStars<R>(acc, sS, dq);
+    __syncthreads();  // all reads of dst (previous src) done
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = t + r * TPE;
+      if (row < m) {
+#pragma unroll
+        for (int p = 0; p < 9; ++p) {
+          dst[row * 9 + p] = dq[r][p];
+          iacc[r][p] += cf.c[d] * dq[r][p];
+        }
+        if (deriv != nullptr && active)
+          for (int p = 0; p < 9; ++p) deriv[e * dsize + doff + p * m + row] = dq[r][p];
+      }
+    }
+    __syncthreads();
+    doff += 9 * m;
+    nIn = m;
+  }
+
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = t + r * TPE;
+      if (row < B)
+#pragma unroll
+        for (int p = 0; p < 9; ++p) integ[e * S::QS + p * B + row] = iacc[r][p];
+    }
+  }
+  if constexpr (WithLocal) {
+    // I as [row][quantity] in sA
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = t + r * TPE;
+      if (row < B)
+#pragma unroll
+        for (int p = 0; p < 9; ++p) xa[row * 9 + p] = iacc[r][p];
+    }
+    __syncthreads();
+    // volume
+    {
+      real acc[3][R][9];
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+          for (int p = 0; p < 9; ++p) acc[j][r][p] = real(0);
+      gemmRows<3, R>(acc, mats + S::vol, static_cast<int64_t>(B) * B, B, xa, B, t, TPE, B);
+      real y[R][9];
+      applyStars<R>(acc, sS, y);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int p = 0; p < 9; ++p) qreg[r][p] += y[r][p];
+    }
+    // face traces times AplusT
+#pragma unroll 1
+    for (int f = 0; f < 4; ++f) {
+      real tr[1][R][9];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int p = 0; p < 9; ++p) tr[0][r][p] = real(0);
+      gemmRows<1, R>(tr, mats + S::fmrt + static_cast<int64_t>(f) * B * B2, 0, B2, xa, B, t, TPE, B2);
+      const bool skip = !active || (faceTypes != nullptr && faceTypes[e * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE);
+      const real* A = aplus + (active ? e : 0) * 324 + 81 * f;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int row = t + r * TPE;
+        if (row < B2) {
+#pragma unroll
+          for (int p = 0; p < 9; ++p) {
+            real w = real(0);
+            if (!skip)
+#pragma unroll
+              for (int q = 0; q < 9; ++q) w += tr[0][r][q] * A[q + 9 * p];
+            xb[(f * B2 + row) * 9 + p] = w;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    {
+      real acc[1][R][9];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int p = 0; p < 9; ++p) acc[0][r][p] = qreg[r][p];
+      gemmRows<1, R>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
+      if (active) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int row = t + r * TPE;
+          if (row < B)
+#pragma unroll
+            for (int p = 0; p < 9; ++p) dofs[e * S::QS + p * B + row] = acc[0][r][p];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// volume + local flux from a given I (Local.cpp:124-160)
+// ---------------------------------------------------------------------------
+template <typename real, int O>
+__global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict__ mats, real* __restrict__ dofs,
+                                              const real* __restrict__ integ, const real* __restrict__ star,
+                                              const real* __restrict__ aplus,
+                                              const uint8_t* __restrict__ faceTypes) {
+  using S = Shape<O>;
+  constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
+  __shared__ real sA[S::EPB * S::QS];
+  __shared__ real sB[S::EPB * S::FS];
+  __shared__ real sStar[S::EPB * 3 * kStarNnz];
+  const int64_t e0 = static_cast<int64_t>(blockIdx.x) * S::EPB;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int slot = wave * S::EPW + lane / TPE;
+  const int t = lane % TPE;
+  const bool laneOk = lane < S::EPW * TPE;
+  const int64_t e = e0 + slot;
+  const bool active = laneOk && e < n;
+  for (int idx = threadIdx.x; idx < S::EPB * S::QS; idx += blockDim.x) {
+    const int s = idx / S::QS, w = idx % S::QS;
+    const int p = w / B, l = w % B;
+    const int64_t ee = e0 + s;
+    sA[s * S::QS + l * 9 + p] = (ee < n) ? integ[ee * S::QS + w] : real(0);
+  }
+  loadStars(sStar, star, e0, n, S::EPB);
+  __syncthreads();
+  const int sl = laneOk ? slot : 0;
+  real* xa = sA + sl * S::QS;
+  real* xb = sB + sl * S::FS;
+  const real* sS = sStar + sl * 3 * kStarNnz;
+
+  real qreg[R][9];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int row = t + r * TPE;
+#pragma unroll
+    for (int p = 0; p < 9; ++p) qreg[r][p] = (active && row < B) ? dofs[e * S::QS + p * B + row] : real(0);
+  }
+  {
+    real acc[3][R][9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int p = 0; p < 9; ++p) acc[j][r][p] = real(0);
+    gemmRows<3, R>(acc, mats + S::vol, static_cast<int64_t>(B) * B, B, xa, B, t, TPE, B);
+    real y[R][9];
+    applyStars<R>(acc, sS, y);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int p = 0; p < 9; ++p) qreg[r][p] += y[r][p];
+  }
+#pragma unroll 1
+  for (int f = 0; f < 4; ++f) {
+    real tr[1][R][9];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int p = 0; p < 9; ++p) tr[0][r][p] = real(0);
+    gemmRows<1, R>(tr, mats + S::fmrt + static_cast<int64_t>(f) * B * B2, 0, B2, xa, B, t, TPE, B2);
+    const bool skip = !active || (faceTypes != nullptr && faceTypes[e * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE);
+    const real* A = aplus + (active ? e : 0) * 324 + 81 * f;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = t + r * TPE;
+      if (row < B2) {
+#pragma unroll
+        for (int p = 0; p < 9; ++p) {
+          real w = real(0);
+          if (!skip)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) w += tr[0][r][q] * A[q + 9 * p];
+          xb[(f * B2 + row) * 9 + p] = w;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  real acc[1][R][9];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int p = 0; p < 9; ++p) acc[0][r][p] = qreg[r][p];
+  gemmRows<1, R>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = t + r * TPE;
+      if (row < B)
+#pragma unroll
+        for (int p = 0; p < 9; ++p) dofs[e * S::QS + p * B + row] = acc[0][r][p];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// neighbour flux (Neighbor.cpp:66-92)
+// ---------------------------------------------------------------------------
+template <typename real, int O>
+__global__ __launch_bounds__(256) void kNeighbor(int64_t n, const real* __restrict__ mats, real* __restrict__ dofs,
+                                                 const real* __restrict__ integ,
+                                                 const int32_t* __restrict__ neighbors,
+                                                 const uint8_t* __restrict__ faceRel,
+                                                 const real* __restrict__ aminus,
+                                                 const uint8_t* __restrict__ faceTypes) {
+  using S = Shape<O>;
+  constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
+  __shared__ real sA[S::EPB * S::QS];
+  __shared__ real sU[S::EPB * B2 * 9];
+  __shared__ real sB[S::EPB * S::FS];
+  const int64_t e0 = static_cast<int64_t>(blockIdx.x) * S::EPB;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int slot = wave * S::EPW + lane / TPE;
+  const int t = lane % TPE;
+  const bool laneOk = lane < S::EPW * TPE;
+  const int64_t e = e0 + slot;
+  const bool active = laneOk && e < n;
+  const int sl = laneOk ? slot : 0;
+  real* xa = sA + sl * S::QS;
+  real* xu = sU + sl * B2 * 9;
+  real* xb = sB + sl * S::FS;
+
+#pragma unroll 1
+  for (int f = 0; f < 4; ++f) {
+    const bool regular = active && (faceTypes == nullptr || faceTypes[e * 4 + f] == SDG_FACE_REGULAR);
+    const int64_t nb = regular ? neighbors[e * 4 + f] : 0;
+    const int j = regular ? faceRel[(e * 4 + f) * 2 + 0] : 0;
+    const int h = regular ? faceRel[(e * 4 + f) * 2 + 1] : 0;
+    if (laneOk) {
+      for (int idx = t; idx < S::QS; idx += TPE) {
+        const int p = idx / B, l = idx % B;
+        xa[l * 9 + p] = regular ? integ[nb * S::QS + idx] : real(0);
+      }
+    }
+    __syncthreads();
+    {
+      real u[1][R][9];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int p = 0; p < 9; ++p) u[0][r][p] = real(0);
+      gemmRows<1, R>(u, mats + S::rt + static_cast<int64_t>(j) * B * B2, 0, B2, xa, B, t, TPE, B2);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int row = t + r * TPE;
+        if (row < B2)
+#pragma unroll
+          for (int p = 0; p < 9; ++p) xu[row * 9 + p] = u[0][r][p];
+      }
+    }
+    __syncthreads();
+    {
+      real v[1][R][9];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int p = 0; p < 9; ++p) v[0][r][p] = real(0);
+      gemmRows<1, R>(v, mats + S::fp + static_cast<int64_t>(h) * B2 * B2, 0, B2, xu, B2, t, TPE, B2);
+      const real* A = aminus + (active ? e : 0) * 324 + 81 * f;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int row = t + r * TPE;
+        if (row < B2) {
+#pragma unroll
+          for (int p = 0; p < 9; ++p) {
+            real w = real(0);
+            if (regular)
+#pragma unroll
+              for (int q = 0; q < 9; ++q) w += v[0][r][q] * A[q + 9 * p];
+            xb[(f * B2 + row) * 9 + p] = w;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  real acc[1][R][9];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int p = 0; p < 9; ++p) acc[0][r][p] = real(0);
+  gemmRows<1, R>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = t + r * TPE;
+      if (row < B)
+#pragma unroll
+        for (int p = 0; p < 9; ++p) dofs[e * S::QS + p * B + row] += acc[0][r][p];
+    }
+  }
+}
+
+// Taylor sum out = sum_i coeffs[i] dQ_i (KernelsAux.cpp:117-184 semantics)
+template <typename real, int O>
+__global__ __launch_bounds__(256) void kTaylor(int64_t n, Coeffs<O, real> cf, const real* __restrict__ deriv,
+                                               int64_t dsize, real* __restrict__ out) {
+  constexpr int B = basis3(O);
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= n * 9 * B) return;
+  const int64_t e = idx / (9 * B);
+  const int w = static_cast<int>(idx % (9 * B));
+  const int p = w / B, k = w % B;
+  real acc = real(0);
+  int64_t off = 0;
+#pragma unroll
+  for (int d = 0; d < O; ++d) {
+    const int m = basis3(O - d);
+    if (k < m) acc += cf.c[d] * deriv[e * dsize + off + p * m + k];
+    off += 9 * m;
+  }
+  out[idx] = acc;
+}
+
+template <typename real, int O>
+Coeffs<O, real> makeCoeffs(const double* c) {
+  Coeffs<O, real> r{};
+  for (int i = 0; i < O; ++i) r.c[i] = static_cast<real>(c[i]);
+  return r;
+}
+
+int64_t derivativesSize(int order) {
+  int64_t s = 0;
+  for (int i = 0; i < order; ++i) s += 9 * basis3(order - i);
+  return s;
+}
+
+template <typename real, int O>
+int aderImpl(const EngineState& st, const double* coeffs, const sdg_layer* L, hipStream_t stream, bool withLocal) {
+  using S = Shape<O>;
+  const int64_t n = L->num_elements;
+  if (n == 0) return SDG_OK;
+  const dim3 grid(static_cast<unsigned>((n + S::EPB - 1) / S::EPB));
+  const auto cf = makeCoeffs<real, O>(coeffs);
+  const auto* mats = static_cast<const real*>(st.matrices);
+  if (withLocal)
+    kAder<real, O, true><<<grid, S::THREADS, 0, stream>>>(
+        n, mats, cf, static_cast<real*>(L->dofs), static_cast<real*>(L->time_integrated),
+        static_cast<real*>(L->derivatives), derivativesSize(O), static_cast<const real*>(L->star),
+        static_cast<const real*>(L->aplus), L->face_types);
+  else
+    kAder<real, O, false><<<grid, S::THREADS, 0, stream>>>(
+        n, mats, cf, static_cast<real*>(L->dofs), static_cast<real*>(L->time_integrated),
+        static_cast<real*>(L->derivatives), derivativesSize(O), static_cast<const real*>(L->star),
+        static_cast<const real*>(L->aplus), L->face_types);
+  SDG_HIP_CHECK(hipGetLastError());
+  return SDG_OK;
+}
+
+template <typename real, int O>
+int localImpl(const EngineState& st, const sdg_layer* L, hipStream_t stream) {
+  using S = Shape<O>;
+  const int64_t n = L->num_elements;
+  if (n == 0) return SDG_OK;
+  const dim3 grid(static_cast<unsigned>((n + S::EPB - 1) / S::EPB));
+  kLocal<real, O><<<grid, S::THREADS, 0, stream>>>(
+      n, static_cast<const real*>(st.matrices), static_cast<real*>(L->dofs),
+      static_cast<const real*>(L->time_integrated), static_cast<const real*>(L->star),
+      static_cast<const real*>(L->aplus), L->face_types);
+  SDG_HIP_CHECK(hipGetLastError());
+  return SDG_OK;
+}
+
+template <typename real, int O>
+int neighborImpl(const EngineState& st, const sdg_layer* L, hipStream_t stream) {
+  using S = Shape<O>;
+  const int64_t n = L->num_elements;
+  if (n == 0) return SDG_OK;
+  const dim3 grid(static_cast<unsigned>((n + S::EPB - 1) / S::EPB));
+  kNeighbor<real, O><<<grid, S::THREADS, 0, stream>>>(
+      n, static_cast<const real*>(st.matrices), static_cast<real*>(L->dofs),
+      static_cast<const real*>(L->time_integrated), L->face_neighbors, L->face_relations,
+      static_cast<const real*>(L->aminus), L->face_types);
+  SDG_HIP_CHECK(hipGetLastError());
+  return SDG_OK;
+}
+
+template <typename real, int O>
+int taylorImpl(const double* coeffs, const void* deriv, void* out, int64_t n, hipStream_t stream) {
+  if (n == 0) return SDG_OK;
+  const int64_t total = n * 9 * basis3(O);
+  const dim3 grid(static_cast<unsigned>((total + 255) / 256));
+  kTaylor<real, O><<<grid, 256, 0, stream>>>(n, makeCoeffs<real, O>(coeffs), static_cast<const real*>(deriv),
+                                              derivativesSize(O), static_cast<real*>(out));
+  SDG_HIP_CHECK(hipGetLastError());
+  return SDG_OK;
+}
+
+template <template <typename, int> class F, typename... Args>
+int dispatch(const EngineState& st, Args&&... args) {
+  const bool dbl = st.arithmetic == SDG_FP64;
+  switch (st.order) {
+#define SDG_CASE(o)                                                            \
+  case o:                                                                      \
+    return dbl ? F<double, o>::run(st, args...) : F<float, o>::run(st, args...);
+    SDG_CASE(2) SDG_CASE(3) SDG_CASE(4) SDG_CASE(5) SDG_CASE(6) SDG_CASE(7)
+#undef SDG_CASE
+    default:
+      setError("unsupported order");
+      return SDG_ERR_UNSUPPORTED;
+  }
+}
+
+template <typename real, int O>
+struct AderOp {
+  static int run(const EngineState& st, const double* c, const sdg_layer* L, hipStream_t s, bool wl) {
+    return aderImpl<real, O>(st, c, L, s, wl);
+  }
+};
+template <typename real, int O>
+struct LocalOp {
+  static int run(const EngineState& st, const sdg_layer* L, hipStream_t s) { return localImpl<real, O>(st, L, s); }
+};
+template <typename real, int O>
+struct NeighborOp {
+  static int run(const EngineState& st, const sdg_layer* L, hipStream_t s) {
+    return neighborImpl<real, O>(st, L, s);
+  }
+};
+template <typename real, int O>
+struct TaylorOp {
+  static int run(const EngineState&, const double* c, const void* d, void* o, int64_t n, hipStream_t s) {
+    return taylorImpl<real, O>(c, d, o, n, s);
+  }
+};
+
+}  // namespace
+
+DeviceMatrixLayout deviceMatrixLayout(int order) {
+  const int B = basis3(order), B2 = basis2(order);
+  DeviceMatrixLayout l{};
+  l.ck = 0;
+  l.vol = l.ck + 3LL * B * B;
+  l.fmrt = l.vol + 3LL * B * B;
+  l.rt = l.fmrt + 4LL * B * B2;
+  l.fp = l.rt + 4LL * B * B2;
+  l.rdm = l.fp + 3LL * B2 * B2;
+  l.total = l.rdm + 4LL * B2 * B;
+  return l;
+}
+
+int launchAder(const EngineState& e, const double* coeffs, const sdg_layer* layer, hipStream_t stream) {
+  return dispatch<AderOp>(e, coeffs, layer, stream, false);
+}
+int launchAderLocal(const EngineState& e, const double* coeffs, const sdg_layer* layer, hipStream_t stream) {
+  return dispatch<AderOp>(e, coeffs, layer, stream, true);
+}
+int launchLocal(const EngineState& e, const sdg_layer* layer, hipStream_t stream) {
+  return dispatch<LocalOp>(e, layer, stream);
+}
+int launchNeighbor(const EngineState& e, const sdg_layer* layer, hipStream_t stream) {
+  return dispatch<NeighborOp>(e, layer, stream);
+}
+int launchTaylor(const EngineState& e, const double* coeffs, const void* derivatives, void* out, int64_t n,
+                 hipStream_t stream) {
+  return dispatch<TaylorOp>(e, coeffs, derivatives, out, n, stream);
+}
+
+}  // namespace sdg

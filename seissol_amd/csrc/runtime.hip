@@ -1,0 +1,398 @@
+// C ABI of the engine: errors, global matrices, synthetic inputs, precompute,
+// engine lifetime, batched kernels and the proxy kernel selection.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/seissol_amd.h"
+#include "cell_local.h"
+#include "engine.h"
+#include "global_matrices.h"
+#include "mesh.h"
+
+namespace sdg {
+
+namespace {
+thread_local std::string gLastError;
+}
+
+void setError(const std::string& msg) { gLastError = msg; }
+
+int hipFail(hipError_t err, const char* what) {
+  gLastError = std::string(what) + ": " + hipGetErrorString(err);
+  return SDG_ERR_HIP;
+}
+
+namespace {
+
+// Reorder the generated matrices into the transposed device layout.
+std::vector<double> deviceMatrices(int order, const double* gm) {
+  const int B = numBasis3(order), B2 = numBasis2(order);
+  const auto g = globalMatrixLayout(order);
+  const auto d = deviceMatrixLayout(order);
+  std::vector<double> out(d.total, 0.0);
+  for (int j = 0; j < 3; ++j)
+    for (int k = 0; k < B; ++k)
+      for (int l = 0; l < B; ++l) {
+        out[d.ck + (j * B + l) * B + k] = -gm[g.kDivMT + (j * B + k) * B + l];
+        out[d.vol + (j * B + l) * B + k] = gm[g.kDivM + (j * B + k) * B + l];
+      }
+  for (int f = 0; f < 4; ++f)
+    for (int m = 0; m < B2; ++m)
+      for (int l = 0; l < B; ++l) {
+        out[d.fmrt + (f * B + l) * B2 + m] = gm[g.fMrT + (f * B2 + m) * B + l];
+        out[d.rt + (f * B + l) * B2 + m] = gm[g.rT + (f * B2 + m) * B + l];
+        out[d.rdm + (f * B2 + m) * B + l] = gm[g.rDivM + (f * B + l) * B2 + m];
+      }
+  for (int h = 0; h < 3; ++h)
+    for (int m = 0; m < B2; ++m)
+      for (int nn = 0; nn < B2; ++nn) out[d.fp + (h * B2 + nn) * B2 + m] = gm[g.fP + (h * B2 + m) * B2 + nn];
+  return out;
+}
+
+int uploadMatrices(EngineState& s, const double* gm) {
+  const auto host = deviceMatrices(s.order, gm);
+  s.layout = deviceMatrixLayout(s.order);
+  const size_t bytes = host.size() * (s.arithmetic == SDG_FP64 ? sizeof(double) : sizeof(float));
+  if (s.matrices == nullptr) SDG_HIP_CHECK(hipMalloc(&s.matrices, bytes));
+  if (s.arithmetic == SDG_FP64) {
+    SDG_HIP_CHECK(hipMemcpy(s.matrices, host.data(), bytes, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(host.begin(), host.end());
+    SDG_HIP_CHECK(hipMemcpy(s.matrices, f.data(), bytes, hipMemcpyHostToDevice));
+  }
+  return SDG_OK;
+}
+
+bool validLayer(const sdg_layer* L, bool needNeighbors) {
+  if (L == nullptr) { setError("layer is NULL"); return false; }
+  if (L->num_elements < 0) { setError("negative element count"); return false; }
+  if (L->num_elements > (int64_t(1) << 31) - 1) { setError("too many elements for 32-bit neighbour ids"); return false; }
+  if (L->num_elements == 0) return true;
+  if (L->dofs == nullptr || L->time_integrated == nullptr) { setError("dofs/time_integrated missing"); return false; }
+  if (needNeighbors && (L->face_neighbors == nullptr || L->face_relations == nullptr || L->aminus == nullptr)) {
+    setError("neighbour data missing");
+    return false;
+  }
+  return true;
+}
+
+// cell-local precompute on the device, one thread per element
+__global__ void kCellLocal(int64_t n, const double* __restrict__ vertices, const double* __restrict__ material,
+                           const int32_t* __restrict__ neighbors, const uint8_t* __restrict__ faceTypes,
+                           double* __restrict__ star, double* __restrict__ aplus, double* __restrict__ aminus) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double v[4][3];
+  for (int q = 0; q < 4; ++q)
+    for (int c = 0; c < 3; ++c) v[q][c] = vertices[(e * 4 + q) * 3 + c];
+  const ElasticMaterial loc{material[3 * e], material[3 * e + 1], material[3 * e + 2]};
+  ElasticMaterial nb[4];
+  int ft[4];
+  for (int f = 0; f < 4; ++f) {
+    const int64_t o = neighbors[e * 4 + f];
+    nb[f] = ElasticMaterial{material[3 * o], material[3 * o + 1], material[3 * o + 2]};
+    ft[f] = faceTypes ? faceTypes[e * 4 + f] : FaceRegular;
+  }
+  cellLocalMatrices(v, loc, nb, ft, star + e * 243, aplus + e * 324, aminus + e * 324);
+}
+
+template <typename Dst>
+__global__ void kConvert(int64_t n, const double* __restrict__ src, Dst* __restrict__ dst) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = static_cast<Dst>(src[i]);
+}
+
+}  // namespace
+}  // namespace sdg
+
+using namespace sdg;
+
+extern "C" {
+
+const char* sdg_last_error(void) { return gLastError.c_str(); }
+int sdg_version(void) { return 1; }
+int sdg_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+size_t sdg_global_matrices_size(int order) {
+  if (order < 1 || order > 8) return 0;
+  return globalMatrixLayout(order).total;
+}
+
+int sdg_global_matrices(int order, double* out) {
+  if (out == nullptr || order < 1 || order > 8) {
+    setError("sdg_global_matrices: order must be in [1, 8] and out non-NULL");
+    return SDG_ERR_INVALID;
+  }
+  return generateGlobalMatrices(order, out) == 0 ? SDG_OK : SDG_ERR_INVALID;
+}
+
+int sdg_basis_values(int order, double xi, double eta, double zeta, double* out) {
+  if (out == nullptr || order < 1 || order > 8) { setError("invalid arguments"); return SDG_ERR_INVALID; }
+  for (int k = 0; k < numBasis3(order); ++k) out[k] = evalTetBasis(order, k, xi, eta, zeta);
+  return SDG_OK;
+}
+
+int sdg_basis_gradients(int order, double xi, double eta, double zeta, double* out) {
+  if (out == nullptr || order < 1 || order > 8) { setError("invalid arguments"); return SDG_ERR_INVALID; }
+  const int B = numBasis3(order);
+  for (int k = 0; k < B; ++k) {
+    double g[3];
+    evalTetBasisGradient(order, k, xi, eta, zeta, g);
+    for (int d = 0; d < 3; ++d) out[d * B + k] = g[d];
+  }
+  return SDG_OK;
+}
+
+int sdg_mesh_proxy(int64_t n, uint64_t seed, double* vertices, int32_t* neighbors, uint8_t* face_relations) {
+  if (n <= 0 || vertices == nullptr || neighbors == nullptr || face_relations == nullptr) {
+    setError("sdg_mesh_proxy: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  proxyMesh(n, seed, vertices, neighbors, face_relations);
+  return SDG_OK;
+}
+
+int64_t sdg_mesh_periodic_size(int nx, int ny, int nz) { return periodicMeshSize(nx, ny, nz); }
+
+int sdg_mesh_periodic(int nx, int ny, int nz, double h, double jitter, uint64_t seed, double* vertices,
+                      int32_t* neighbors, uint8_t* face_relations) {
+  if (vertices == nullptr || neighbors == nullptr || face_relations == nullptr || jitter < 0 || jitter > 0.2) {
+    setError("sdg_mesh_periodic: invalid arguments (jitter must be in [0, 0.2])");
+    return SDG_ERR_INVALID;
+  }
+  const int rc = periodicMesh(nx, ny, nz, h, jitter, seed, vertices, neighbors, face_relations);
+  if (rc != 0) {
+    setError("sdg_mesh_periodic: dimensions must be even and >= 4 (code " + std::to_string(rc) + ")");
+    return SDG_ERR_INVALID;
+  }
+  return SDG_OK;
+}
+
+int sdg_random_materials(int64_t n, uint64_t seed, double rho_min, double rho_max, double vp, double vs,
+                         double* material) {
+  if (n < 0 || material == nullptr || rho_min <= 0 || rho_max < rho_min || vs <= 0 || vp <= vs * std::sqrt(2.0)) {
+    setError("sdg_random_materials: invalid arguments (need vp > sqrt(2) vs > 0)");
+    return SDG_ERR_INVALID;
+  }
+  randomMaterials(n, seed, MaterialRange{rho_min, rho_max, vp, vs}, material);
+  return SDG_OK;
+}
+
+double sdg_cfl_timestep(int64_t n, int order, double cfl, const double* vertices, const double* material) {
+  if (n <= 0 || vertices == nullptr || material == nullptr) return -1.0;
+  return minCflTimestep(n, order, cfl, vertices, material);
+}
+
+int sdg_uniform_dofs(int64_t count, uint64_t seed, double* out) {
+  if (count < 0 || out == nullptr) { setError("invalid arguments"); return SDG_ERR_INVALID; }
+  uniformDofs(count, seed, out);
+  return SDG_OK;
+}
+
+int sdg_monomial_integrate(int order, double t0, double t1, double* coeffs) {
+  if (order < 1 || coeffs == nullptr) { setError("invalid arguments"); return SDG_ERR_INVALID; }
+  double p0 = t0, p1 = t1, fact = 1.0;
+  for (int i = 0; i < order; ++i) {
+    fact *= (i + 1);
+    coeffs[i] = (p1 - p0) / fact;
+    p0 *= t0;
+    p1 *= t1;
+  }
+  return SDG_OK;
+}
+
+int sdg_cell_local_matrices(int64_t n, const double* vertices, const double* material, const int32_t* neighbors,
+                            const uint8_t* face_types, double* star, double* aplus, double* aminus, void* stream) {
+  if (n < 0 || (n > 0 && (!vertices || !material || !neighbors || !star || !aplus || !aminus))) {
+    setError("sdg_cell_local_matrices: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  if (n == 0) return SDG_OK;
+  const unsigned grid = static_cast<unsigned>((n + 63) / 64);
+  kCellLocal<<<grid, 64, 0, static_cast<hipStream_t>(stream)>>>(n, vertices, material, neighbors, face_types,
+                                                                  star, aplus, aminus);
+  SDG_HIP_CHECK(hipGetLastError());
+  return SDG_OK;
+}
+
+int64_t sdg_derivatives_size(int order) {
+  int64_t s = 0;
+  for (int i = 0; i < order; ++i) s += 9LL * numBasis3(order - i);
+  return s;
+}
+
+int sdg_engine_create(sdg_engine** engine, int order, int arithmetic) {
+  if (engine == nullptr) { setError("engine is NULL"); return SDG_ERR_INVALID; }
+  *engine = nullptr;
+  if (order < kMinOrder || order > kMaxOrder) {
+    setError("sdg_engine_create: order must be in [2, 7]");
+    return SDG_ERR_UNSUPPORTED;
+  }
+  if (arithmetic != SDG_FP64 && arithmetic != SDG_FP32 && arithmetic != SDG_BF16X3) {
+    setError("sdg_engine_create: unknown arithmetic");
+    return SDG_ERR_INVALID;
+  }
+  auto* e = new sdg_engine();
+  e->s.order = order;
+  e->s.arithmetic = arithmetic;
+  std::vector<double> gm(globalMatrixLayout(order).total);
+  generateGlobalMatrices(order, gm.data());
+  const int rc = uploadMatrices(e->s, gm.data());
+  if (rc != SDG_OK) {
+    if (e->s.matrices) (void)hipFree(e->s.matrices);
+    delete e;
+    return rc;
+  }
+  *engine = e;
+  return SDG_OK;
+}
+
+int sdg_engine_set_global_matrices(sdg_engine* engine, const double* host_matrices) {
+  if (engine == nullptr || host_matrices == nullptr) { setError("invalid arguments"); return SDG_ERR_INVALID; }
+  return uploadMatrices(engine->s, host_matrices);
+}
+
+int sdg_engine_destroy(sdg_engine* engine) {
+  if (engine == nullptr) return SDG_OK;
+  if (engine->s.matrices) SDG_HIP_CHECK(hipFree(engine->s.matrices));
+  delete engine;
+  return SDG_OK;
+}
+
+int sdg_compute_batched_ader(sdg_engine* engine, const double* coeffs, double /*time_step_width*/,
+                             const sdg_layer* layer, void* stream) {
+  if (engine == nullptr || coeffs == nullptr || !validLayer(layer, false)) return SDG_ERR_INVALID;
+  if (layer->num_elements > 0 && layer->star == nullptr) { setError("star matrices missing"); return SDG_ERR_INVALID; }
+  return launchAder(engine->s, coeffs, layer, static_cast<hipStream_t>(stream));
+}
+
+int sdg_compute_batched_ader_local(sdg_engine* engine, const double* coeffs, double /*time_step_width*/,
+                                   const sdg_layer* layer, void* stream) {
+  if (engine == nullptr || coeffs == nullptr || !validLayer(layer, false)) return SDG_ERR_INVALID;
+  if (layer->num_elements > 0 && (layer->star == nullptr || layer->aplus == nullptr)) {
+    setError("star/aplus matrices missing");
+    return SDG_ERR_INVALID;
+  }
+  return launchAderLocal(engine->s, coeffs, layer, static_cast<hipStream_t>(stream));
+}
+
+int sdg_evaluate_batched(sdg_engine* engine, const double* coeffs, const void* derivatives, void* time_evaluated,
+                         int64_t num_elements, void* stream) {
+  if (engine == nullptr || coeffs == nullptr || num_elements < 0 ||
+      (num_elements > 0 && (derivatives == nullptr || time_evaluated == nullptr))) {
+    setError("sdg_evaluate_batched: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  return launchTaylor(engine->s, coeffs, derivatives, time_evaluated, num_elements,
+                      static_cast<hipStream_t>(stream));
+}
+
+int sdg_compute_batched_integral(sdg_engine* engine, const sdg_layer* layer, void* stream) {
+  if (engine == nullptr || !validLayer(layer, false)) return SDG_ERR_INVALID;
+  if (layer->num_elements > 0 && (layer->star == nullptr || layer->aplus == nullptr)) {
+    setError("star/aplus matrices missing");
+    return SDG_ERR_INVALID;
+  }
+  return launchLocal(engine->s, layer, static_cast<hipStream_t>(stream));
+}
+
+int sdg_compute_batched_neighbors_integral(sdg_engine* engine, const sdg_layer* layer, void* stream) {
+  if (engine == nullptr || !validLayer(layer, true)) return SDG_ERR_INVALID;
+  return launchNeighbor(engine->s, layer, static_cast<hipStream_t>(stream));
+}
+
+int sdg_proxy_kernel_id(const char* name) {
+  if (name == nullptr) return -1;
+  const char* names[] = {"all", "local", "neigh", "ader", "localwoader"};
+  for (int i = 0; i < 5; ++i)
+    if (std::strcmp(name, names[i]) == 0) return i;
+  return -1;
+}
+
+int sdg_proxy_run(sdg_engine* engine, int kernel_id, int steps, double dt, const sdg_layer* layer, void* stream) {
+  if (engine == nullptr || steps < 0 || kernel_id < 0 || kernel_id > 4 || !validLayer(layer, kernel_id == 0 || kernel_id == 2)) {
+    if (engine == nullptr || steps < 0 || kernel_id < 0 || kernel_id > 4) setError("sdg_proxy_run: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  std::vector<double> coeffs(engine->s.order);
+  sdg_monomial_integrate(engine->s.order, 0.0, dt, coeffs.data());
+  auto st = static_cast<hipStream_t>(stream);
+  for (int s = 0; s < steps; ++s) {
+    int rc = SDG_OK;
+    switch (kernel_id) {
+      case 0:
+        rc = launchAderLocal(engine->s, coeffs.data(), layer, st);
+        if (rc == SDG_OK) rc = launchNeighbor(engine->s, layer, st);
+        break;
+      case 1: rc = launchAderLocal(engine->s, coeffs.data(), layer, st); break;
+      case 2: rc = launchNeighbor(engine->s, layer, st); break;
+      case 3: rc = launchAder(engine->s, coeffs.data(), layer, st); break;
+      case 4: rc = launchLocal(engine->s, layer, st); break;
+    }
+    if (rc != SDG_OK) return rc;
+  }
+  return SDG_OK;
+}
+
+int sdg_upload_reals(sdg_engine* engine, void* device_dst, const double* host_src, int64_t count, void* stream) {
+  if (engine == nullptr || count < 0 || (count > 0 && (!device_dst || !host_src))) {
+    setError("sdg_upload_reals: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  auto st = static_cast<hipStream_t>(stream);
+  if (engine->s.arithmetic == SDG_FP64) {
+    SDG_HIP_CHECK(hipMemcpyAsync(device_dst, host_src, count * sizeof(double), hipMemcpyHostToDevice, st));
+    SDG_HIP_CHECK(hipStreamSynchronize(st));
+  } else {
+    std::vector<float> f(host_src, host_src + count);
+    SDG_HIP_CHECK(hipMemcpyAsync(device_dst, f.data(), count * sizeof(float), hipMemcpyHostToDevice, st));
+    SDG_HIP_CHECK(hipStreamSynchronize(st));
+  }
+  return SDG_OK;
+}
+
+int sdg_download_reals(sdg_engine* engine, double* host_dst, const void* device_src, int64_t count, void* stream) {
+  if (engine == nullptr || count < 0 || (count > 0 && (!device_src || !host_dst))) {
+    setError("sdg_download_reals: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  auto st = static_cast<hipStream_t>(stream);
+  if (engine->s.arithmetic == SDG_FP64) {
+    SDG_HIP_CHECK(hipMemcpyAsync(host_dst, device_src, count * sizeof(double), hipMemcpyDeviceToHost, st));
+    SDG_HIP_CHECK(hipStreamSynchronize(st));
+  } else {
+    std::vector<float> f(count);
+    SDG_HIP_CHECK(hipMemcpyAsync(f.data(), device_src, count * sizeof(float), hipMemcpyDeviceToHost, st));
+    SDG_HIP_CHECK(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < count; ++i) host_dst[i] = f[i];
+  }
+  return SDG_OK;
+}
+
+int sdg_convert_fp64_to_real(sdg_engine* engine, void* device_dst, const double* device_src, int64_t count,
+                             void* stream) {
+  if (engine == nullptr || count < 0 || (count > 0 && (!device_dst || !device_src))) {
+    setError("sdg_convert_fp64_to_real: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  if (count == 0) return SDG_OK;
+  auto st = static_cast<hipStream_t>(stream);
+  if (engine->s.arithmetic == SDG_FP64) {
+    SDG_HIP_CHECK(hipMemcpyAsync(device_dst, device_src, count * sizeof(double), hipMemcpyDeviceToDevice, st));
+  } else {
+    const unsigned grid = static_cast<unsigned>((count + 255) / 256);
+    kConvert<float><<<grid, 256, 0, st>>>(count, device_src, static_cast<float*>(device_dst));
+    SDG_HIP_CHECK(hipGetLastError());
+  }
+  return SDG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+"""Proxy harness: synthetic layers and the proxy kernel selection.
+
+Mirrors the reference proxy (src/Proxy/Allocator.cpp:49-134 for the data,
+src/Proxy/Common.cpp:22-41 for the kernel names, src/Proxy/Runner.cpp:57-154
+for the timing protocol) with physical synthetic data:
+
+* ``mesh="proxy"``: independent random affine tetrahedra (det J in [0.5, 2]),
+  random face neighbours and random face relations, as the reference proxy
+  fabricates them;
+* ``mesh="periodic"``: a conforming periodic jittered tetrahedral box whose
+  neighbours and face relations follow from shared vertices.
+
+Materials rho ~ U[rho_min, rho_max], vp, vs fixed; DOFs ~ U[-1, 1]; dt from the
+CFL condition (src/Initializer/TimeStepping/GlobalTimestep.cpp:27-47).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi
+from ._capi import Layer, check, lib, ptr
+from .kernels import Engine, derivatives_size, num_basis
+
+KERNELS = ("all", "local", "neigh", "ader", "localwoader")
+
+
+@dataclass
+class Setup:
+    """Host-side FP64 description of a layer of cells."""
+
+    order: int
+    n: int
+    vertices: np.ndarray        # [n][4][3]
+    material: np.ndarray        # [n][3] (rho, lambda, mu)
+    neighbors: np.ndarray       # [n][4] int32
+    face_relations: np.ndarray  # [n][4][2] uint8
+    face_types: np.ndarray      # [n][4] uint8
+    dofs: np.ndarray            # [n][9][B]
+    dt: float
+    seed: int
+    mesh: str
+    dims: tuple = field(default=())
+
+
+def periodic_dims_for(n_target: int) -> tuple:
+    """Even box dimensions (>= 4) whose 5-tet mesh has about n_target cells."""
+    cubes = max(64, n_target // 5)
+    best = None
+    for nx in range(4, 400, 2):
+        for ny in range(nx, 400, 2):
+            nz = cubes // (nx * ny)
+            nz -= nz % 2
+            if nz < ny:
+                break
+            got = 5 * nx * ny * nz
+            if best is None or abs(got - n_target) < abs(best[0] - n_target):
+                best = (got, (nx, ny, nz))
+    return best[1]
+
+
+def make_setup(order: int, n: int | None = None, mesh: str = "proxy", seed: int = 1234,
+               dims: tuple | None = None, rho=(2000.0, 3000.0), vp: float = 6000.0, vs: float = 3464.0,
+               cfl: float = 0.5, jitter: float = 0.1, h: float = 100.0) -> Setup:
+    L = lib()
+    if mesh == "proxy":
+        if n is None or n <= 0:
+            raise ValueError("proxy mesh needs n > 0")
+        vertices = np.zeros((n, 4, 3))
+        neighbors = np.zeros((n, 4), dtype=np.int32)
+        rel = np.zeros((n, 4, 2), dtype=np.uint8)
+        check(L.sdg_mesh_proxy(n, seed, vertices.ctypes.data, neighbors.ctypes.data, rel.ctypes.data),
+              "sdg_mesh_proxy")
+        dims = ()
+    elif mesh == "periodic":
+        dims = tuple(dims) if dims else periodic_dims_for(n)
+        n = int(L.sdg_mesh_periodic_size(*dims))
+        vertices = np.zeros((n, 4, 3))
+        neighbors = np.zeros((n, 4), dtype=np.int32)
+        rel = np.zeros((n, 4, 2), dtype=np.uint8)
+        check(L.sdg_mesh_periodic(*dims, h, jitter, seed, vertices.ctypes.data, neighbors.ctypes.data,
+                                  rel.ctypes.data), "sdg_mesh_periodic")
+    else:
+        raise ValueError(f"unknown mesh kind {mesh!r}")
+    material = np.zeros((n, 3))
+    check(L.sdg_random_materials(n, seed + 1, rho[0], rho[1], vp, vs, material.ctypes.data),
+          "sdg_random_materials")
+    B = num_basis(order)
+    dofs = np.zeros((n, 9, B))
+    check(L.sdg_uniform_dofs(dofs.size, seed + 2, dofs.ctypes.data), "sdg_uniform_dofs")
+    dt = float(L.sdg_cfl_timestep(n, order, cfl, vertices.ctypes.data, material.ctypes.data))
+    face_types = np.zeros((n, 4), dtype=np.uint8)
+    return Setup(order, n, vertices, material, neighbors, rel, face_types, dofs, dt, seed, mesh, dims)
+
+
+class DeviceLayer:
+    """Device arrays of one layer plus the ``sdg_layer`` view handed to the C ABI.
+
+    The per-element operators are computed on the device in FP64
+    (``sdg_cell_local_matrices``) and converted to the engine's real type.
+    """
+
+    def __init__(self, engine: Engine, setup: Setup, with_derivatives: bool = False, device: str = "cuda"):
+        import torch
+
+        self.engine = engine
+        self.setup = setup
+        n, B = setup.n, num_basis(setup.order)
+        rt = engine.real_dtype
+        dev = torch.device(device)
+        f64 = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        self.vertices = f64(setup.vertices)
+        self.material = f64(setup.material)
+        self.neighbors = f64(setup.neighbors)
+        self.face_relations = f64(setup.face_relations)
+        self.face_types = f64(setup.face_types)
+        star64 = torch.zeros((n, 3, 81), dtype=torch.float64, device=dev)
+        aplus64 = torch.zeros((n, 4, 81), dtype=torch.float64, device=dev)
+        aminus64 = torch.zeros((n, 4, 81), dtype=torch.float64, device=dev)
+        stream = torch.cuda.current_stream().cuda_stream
+        check(lib().sdg_cell_local_matrices(n, ptr(self.vertices), ptr(self.material), ptr(self.neighbors),
+                                            ptr(self.face_types), ptr(star64), ptr(aplus64), ptr(aminus64),
+                                            stream), "sdg_cell_local_matrices")
+        self.star64, self.aplus64, self.aminus64 = star64, aplus64, aminus64
+        if rt == torch.float64:
+            self.star, self.aplus, self.aminus = star64, aplus64, aminus64
+        else:
+            self.star, self.aplus, self.aminus = star64.to(rt), aplus64.to(rt), aminus64.to(rt)
+        self.dofs = torch.from_numpy(setup.dofs).to(dev).to(rt).contiguous()
+        self.time_integrated = torch.zeros((n, 9, B), dtype=rt, device=dev)
+        self.derivatives = (torch.zeros((n, derivatives_size(setup.order)), dtype=rt, device=dev)
+                            if with_derivatives else None)
+        self.view = Layer(n, ptr(self.dofs), ptr(self.time_integrated), ptr(self.derivatives), ptr(self.star),
+                          ptr(self.aplus), ptr(self.aminus), ptr(self.neighbors), ptr(self.face_relations),
+                          ptr(self.face_types))
+
+    def reset_dofs(self, dofs: np.ndarray):
+        import torch
+        self.dofs.copy_(torch.from_numpy(dofs).to(self.dofs.device).to(self.dofs.dtype))
+
+
+def kernel_id(name: str) -> int:
+    k = lib().sdg_proxy_kernel_id(name.encode())
+    if k < 0:
+        raise ValueError(f"unknown proxy kernel {name!r}; expected one of {KERNELS}")
+    return k
+
+
+def run_proxy(engine: Engine, layer: DeviceLayer, kernel: str, steps: int, dt: float, stream=None):
+    """Enqueue `steps` repetitions of proxy kernel `kernel` (no synchronisation)."""
+    import torch
+    s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+    check(lib().sdg_proxy_run(engine.handle, kernel_id(kernel), steps, float(dt), C.byref(layer.view), s),
+          "sdg_proxy_run")

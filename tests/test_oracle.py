@@ -1,0 +1,136 @@
+"""Pins of the CPU FP64 oracle: reference golden data, invariants and the
+closed-form exactness of one ADER-DG step for polynomial states."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import (PolyField, coefficient_matrices, evolve, interior_elements, project, rel_l2)
+from seissol_amd.kernels import global_matrices, monomial_integrate, num_basis
+from seissol_amd.proxy import make_setup
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _build_oracle():
+    oracle.build()
+
+
+def test_godunov_golden():
+    g = json.loads((GOLDEN / "godunov_elastic.json").read_text())
+    tol = g["tolerance"]
+    m1, m2 = np.array(g["MaterialVal1"]), np.array(g["MaterialVal2"])
+    for (loc, nb, tag) in ((m1, m1, "Homogeneous"), (m1, m2, "Heterogeneous")):
+        ql, qn = oracle.godunov(loc, nb)
+        for got, name in ((ql, f"Solution{tag}Local"), (qn, f"Solution{tag}Neighbor")):
+            ref = np.array(g[name])
+            # relative Frobenius comparison (tests/Model/GodunovState.t.h:35-44)
+            assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref), name
+
+
+def test_godunov_invariants_random_materials():
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        def mat():
+            rho = rng.uniform(1500, 3500)
+            vs = rng.uniform(1000, 4000)
+            vp = vs * rng.uniform(1.5, 2.2)
+            mu = rho * vs * vs
+            return np.array([rho, rho * vp * vp - 2 * mu, mu])
+        ql, qn = oracle.godunov(mat(), mat())
+        eps = 1e2 * np.finfo(float).eps
+        assert np.linalg.norm(ql + qn - np.eye(9)) <= eps * 3
+        assert np.linalg.norm(qn @ qn - qn) <= eps * np.linalg.norm(qn)
+
+
+def test_star_matrices_on_reference_tet():
+    # on the reference tetrahedron grad(xi, eta, zeta) = I, so star_d = A_d^T
+    v = np.array([[[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]]], float)
+    mat = np.array([[2700.0, 3.2e10, 3.3e10]])
+    star, ap, am = oracle.cell_local_matrices(v, mat, np.zeros((1, 4), np.int32))
+    A = coefficient_matrices(*mat[0])
+    for d in range(3):
+        np.testing.assert_allclose(star[0, d].reshape(9, 9).T, A[d].T, rtol=1e-14, atol=1e-20)
+
+
+def _poly_step(order, dims=(6, 6, 6), relations_override=None, seed=3):
+    setup = make_setup(order, mesh="periodic", dims=dims, seed=seed, rho=(2500.0, 2500.0), h=100.0, jitter=0.12)
+    gm = global_matrices(order)["flat"]
+    star, ap, am = oracle.cell_local_matrices(setup.vertices, setup.material, setup.neighbors)
+    A = coefficient_matrices(*setup.material[0])
+    rng = np.random.default_rng(seed)
+    # per-quantity amplitudes of a physically scaled state (stresses ~ rho c^2 * strain)
+    amp = np.array([1e6] * 6 + [1.0] * 3)
+    field = PolyField(order, scale=dims[0] * 100.0, rng=rng, amplitude=amp)
+    dt = setup.dt
+    q0 = project(order, setup.vertices, field)
+    coeffs = monomial_integrate(order, 0.0, dt)
+    rel = setup.face_relations if relations_override is None else relations_override(setup.face_relations)
+    q1, I = oracle.proxy_all(order, gm, 1, coeffs, star, ap, am, setup.neighbors, rel, q0)
+    exact_I = project(order, setup.vertices, evolve(field, A, dt, integrate=True))
+    exact_q1 = project(order, setup.vertices, evolve(field, A, dt))
+    inner = interior_elements(setup)
+    return q0, q1, I, exact_q1, exact_I, inner
+
+
+@pytest.mark.parametrize("order", [2, 3, 4, 5, 6])
+def test_one_step_exact_for_polynomial_states(order):
+    q0, q1, I, exact_q1, exact_I, inner = _poly_step(order)
+    assert len(inner) > 50
+    # time integral (CK + Taylor) is exact everywhere
+    assert rel_l2(I, exact_I) < 1e-11
+    # the full update is exact away from the periodic wrap (continuous traces)
+    change = np.linalg.norm(exact_q1[inner] - q0[inner])
+    err = np.linalg.norm(q1[inner] - exact_q1[inner])
+    assert err < 1e-9 * change, (err, change)
+
+
+def test_planted_orientation_mistake_is_caught():
+    # a wrong orientation convention (h -> (h+1) % 3) must break exactness
+    def wrong(rel):
+        r = rel.copy()
+        r[..., 1] = (r[..., 1] + 1) % 3
+        return r
+    q0, q1, I, exact_q1, exact_I, inner = _poly_step(4, relations_override=wrong)
+    change = np.linalg.norm(exact_q1[inner] - q0[inner])
+    err = np.linalg.norm(q1[inner] - exact_q1[inner])
+    assert err > 1e-3 * change
+
+
+def test_planted_side_mistake_is_caught():
+    def wrong(rel):
+        r = rel.copy()
+        r[..., 0] = (r[..., 0] + 1) % 4
+        return r
+    q0, q1, I, exact_q1, exact_I, inner = _poly_step(3, relations_override=wrong)
+    change = np.linalg.norm(exact_q1[inner] - q0[inner])
+    err = np.linalg.norm(q1[inner] - exact_q1[inner])
+    assert err > 1e-3 * change
+
+
+def test_taylor_matches_ader_integral():
+    order = 5
+    setup = make_setup(order, n=64, mesh="proxy", seed=11)
+    gm = global_matrices(order)["flat"]
+    star, ap, am = oracle.cell_local_matrices(setup.vertices, setup.material, setup.neighbors)
+    coeffs = monomial_integrate(order, 0.0, setup.dt)
+    I, d = oracle.ader(order, gm, coeffs, setup.dofs, star, want_derivatives=True)
+    np.testing.assert_allclose(oracle.taylor(order, coeffs, d), I, rtol=1e-13, atol=1e-13 * np.abs(I).max())
+    # derivative block 0 is Q itself
+    B = num_basis(order)
+    np.testing.assert_array_equal(d[:, :9 * B].reshape(-1, 9, B), setup.dofs)
+
+
+def test_dynamic_rupture_faces_have_no_local_or_neighbour_flux():
+    order = 3
+    setup = make_setup(order, n=32, mesh="proxy", seed=2)
+    gm = global_matrices(order)["flat"]
+    ft = np.ones((setup.n, 4), np.uint8)
+    star, ap, am = oracle.cell_local_matrices(setup.vertices, setup.material, setup.neighbors, ft)
+    assert np.abs(ap).max() == 0.0  # fluxScale = 0 for AplusT on DR faces
+    I = oracle.ader(order, gm, monomial_integrate(order, 0, setup.dt), setup.dofs, star)
+    q = oracle.neighbor(order, gm, I, setup.neighbors, setup.face_relations, am, setup.dofs, ft)
+    np.testing.assert_array_equal(q, setup.dofs)
