@@ -1,0 +1,167 @@
+"""Benchmark of the ADER-DG element update (proxy kernel `all`) on one GPU.
+
+Workload (default): elastic ORDER 6 (56 basis functions x 9 quantities),
+400,000 tetrahedra of a conforming periodic jittered box (40 x 40 x 50 cubes,
+5 tets each), rho ~ U[2000, 3000], vp 6000, vs 3464, DOFs ~ U[-1, 1], dt from
+CFL 0.5, global time stepping; one step = ader + local integral + neighbour
+integral of every element.  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line (rank 0).  Besides the throughput it reports the roofline
+of the dominant kernel (algorithmic bytes / measured average launch time, HIP
+events on the launch stream) and the CPU FP64 oracle timed on a bounded sample
+of the same workload on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "element-updates/s ORDER=6 elastic, chosen vs plain FP64 GPU at L2 err <=1e-5"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(order: int, real_bytes: int) -> dict:
+    """Per element and pass: DOF-block traffic plus per-element operators
+    (star as 3x24 non-zeros, 4 AplusT, 4 AminusT); global matrices excluded."""
+    B = order * (order + 1) * (order + 2) // 6
+    q = 9 * B
+    local = (3 * q + 3 * 24 + 4 * 81) * real_bytes      # read Q, write I, write Q, star, AplusT
+    neigh = (3 * q + 4 * 81) * real_bytes               # read Q, read I (once), write Q, AminusT
+    return {"local": local, "neigh": neigh, "update": local + neigh}
+
+
+def cpu_baseline(order: int, elements: int, steps: int, seed: int) -> dict:
+    import oracle
+    from seissol_amd.kernels import global_matrices, monomial_integrate
+    from seissol_amd.proxy import make_setup
+
+    oracle.build()
+    setup = make_setup(order, n=elements, mesh="periodic", seed=seed)
+    gm = global_matrices(order)["flat"]
+    star, ap, am = oracle.cell_local_matrices(setup.vertices, setup.material, setup.neighbors)
+    coeffs = monomial_integrate(order, 0.0, setup.dt)
+    oracle.proxy_all(order, gm, 1, coeffs, star, ap, am, setup.neighbors, setup.face_relations, setup.dofs)
+    t0 = time.perf_counter()
+    oracle.proxy_all(order, gm, steps, coeffs, star, ap, am, setup.neighbors, setup.face_relations, setup.dofs)
+    sec = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": setup.n * steps / sec, "unit": "element-updates/s", "cores": threads, "kind": "port",
+            "sample": f"ORDER {order}, {setup.n} elements x {steps} steps, proxy `all`, FP64 OpenMP oracle "
+                      f"(CPU restatement, not the reference binary), {sec:.1f} s"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--order", type=int, default=6)
+    ap.add_argument("--elements", type=int, default=400_000)
+    ap.add_argument("--form", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--mesh", choices=["periodic", "proxy"], default="periodic")
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--cpu-elements", type=int, default=20_000)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
+    args = ap.parse_args(argv)
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
+        return 0
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the element update would shard)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from seissol_amd.kernels import Engine
+    from seissol_amd.proxy import DeviceLayer, make_setup, run_proxy
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    dims = (40, 40, 50) if (args.mesh == "periodic" and args.elements == 400_000) else None
+    t0 = time.perf_counter()
+    setup = make_setup(args.order, n=args.elements, mesh=args.mesh, seed=args.seed, dims=dims)
+    log(f"setup {setup.n} elements ({args.mesh}) in {time.perf_counter() - t0:.1f} s, dt={setup.dt:.3e}")
+    eng = Engine(args.order, args.form)
+    layer = DeviceLayer(eng, setup)
+    torch.cuda.synchronize()
+    log("device layer ready")
+
+    run_proxy(eng, layer, "all", args.warmup, setup.dt)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    run_proxy(eng, layer, "all", args.steps, setup.dt)
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end)
+    value = setup.n * args.steps / (ms / 1e3)
+    log(f"timed {args.steps} steps: {ms:.1f} ms -> {value / 1e6:.2f} M element-updates/s")
+    finite = bool(torch.isfinite(layer.dofs).all().item())
+
+    # per-kernel launch durations (same stream, HIP events)
+    per_kernel = {}
+    reps = max(5, min(args.steps, 50))
+    for name in ("local", "neigh"):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        run_proxy(eng, layer, name, reps, setup.dt)
+        e.record()
+        torch.cuda.synchronize()
+        per_kernel[name] = s.elapsed_time(e) / reps
+    real_bytes = 8 if args.form == "fp64" else 4
+    ab = algorithmic_bytes(args.order, real_bytes)
+    dom = max(per_kernel, key=per_kernel.get)
+    achieved = ab[dom] * setup.n / (per_kernel[dom] / 1e3) / 1e9
+    log(f"per launch: local {per_kernel['local']:.3f} ms, neigh {per_kernel['neigh']:.3f} ms")
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "element-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.form == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": f"elastic ADER-DG element update (proxy `all`), ORDER {args.order}, "
+                               f"{setup.n} tetrahedra, {args.mesh} mesh, {args.form}, global time stepping",
+                   "order": args.order, "elements": setup.n, "form": args.form, "mesh": args.mesh,
+                   "seed": args.seed, "dt": setup.dt, "finite": finite},
+        "roofline": {"bound": "hbm", "kernel": f"proxy `{dom}` ({'ader+volume+local flux' if dom == 'local' else 'neighbour flux'})",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_element": ab[dom],
+                     "launch_ms": per_kernel[dom], "other_launch_ms": {k: v for k, v in per_kernel.items() if k != dom}},
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline(args.order, args.cpu_elements, args.cpu_steps, args.seed)
+        log(f"cpu baseline {result['cpu_baseline']['value']:.0f} element-updates/s")
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

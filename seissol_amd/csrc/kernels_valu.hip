@@ -122,7 +122,10 @@ __device__ inline void gemmRows(real (&acc)[J][R][9], const real* __restrict__ A
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int row = t + r * tpe;
-        const real a = row < M ? AT[j * strideJ + static_cast<int64_t>(l) * ldAT + row] : real(0);
+        // clamped address: lanes past the last row read a valid entry and drop it
+        const int rr = row < M ? row : M - 1;
+        const real av = AT[j * strideJ + static_cast<int64_t>(l) * ldAT + rr];
+        const real a = row < M ? av : real(0);
 #pragma unroll
         for (int p = 0; p < 9; ++p) acc[j][r][p] += a * x[p];
       }
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(256) void kAder(int64_t n, const real* __restrict__
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int row = t + r * TPE;
-      if (row < m) {
+      if (laneOk && row < m) {
 #pragma unroll
         for (int p = 0; p < 9; ++p) {
           dst[row * 9 + p] = dq[r][p];
@@ -239,7 +242,7 @@ __global__ __launch_bounds__(256) void kAder(int64_t n, const real* __restrict__
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int row = t + r * TPE;
-      if (row < B)
+      if (laneOk && row < B)
 #pragma unroll
         for (int p = 0; p < 9; ++p) xa[row * 9 + p] = iacc[r][p];
     }
@@ -275,7 +278,7 @@ __global__ __launch_bounds__(256) void kAder(int64_t n, const real* __restrict__
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int row = t + r * TPE;
-        if (row < B2) {
+        if (laneOk && row < B2) {
 #pragma unroll
           for (int p = 0; p < 9; ++p) {
             real w = real(0);
@@ -378,7 +381,7 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int row = t + r * TPE;
-      if (row < B2) {
+      if (laneOk && row < B2) {
 #pragma unroll
         for (int p = 0; p < 9; ++p) {
           real w = real(0);
@@ -440,8 +443,9 @@ __global__ __launch_bounds__(256) void kNeighbor(int64_t n, const real* __restri
   for (int f = 0; f < 4; ++f) {
     const bool regular = active && (faceTypes == nullptr || faceTypes[e * 4 + f] == SDG_FACE_REGULAR);
     const int64_t nb = regular ? neighbors[e * 4 + f] : 0;
-    const int j = regular ? faceRel[(e * 4 + f) * 2 + 0] : 0;
-    const int h = regular ? faceRel[(e * 4 + f) * 2 + 1] : 0;
+    // clamp so malformed relations can never index outside the matrix table
+    const int j = regular ? min(static_cast<int>(faceRel[(e * 4 + f) * 2 + 0]), 3) : 0;
+    const int h = regular ? min(static_cast<int>(faceRel[(e * 4 + f) * 2 + 1]), 2) : 0;
     if (laneOk) {
       for (int idx = t; idx < S::QS; idx += TPE) {
         const int p = idx / B, l = idx % B;
@@ -459,7 +463,7 @@ __global__ __launch_bounds__(256) void kNeighbor(int64_t n, const real* __restri
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int row = t + r * TPE;
-        if (row < B2)
+        if (laneOk && row < B2)
 #pragma unroll
           for (int p = 0; p < 9; ++p) xu[row * 9 + p] = u[0][r][p];
       }
@@ -476,7 +480,7 @@ __global__ __launch_bounds__(256) void kNeighbor(int64_t n, const real* __restri
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int row = t + r * TPE;
-        if (row < B2) {
+        if (laneOk && row < B2) {
 #pragma unroll
           for (int p = 0; p < 9; ++p) {
             real w = real(0);

@@ -1,0 +1,196 @@
+"""Parity of the HIP element-update path with the FP64 oracle (GPU tests).
+
+Every call goes through the C ABI (ctypes).  FP64 results must agree with the
+oracle to rel. L2 <= 1e-12 (reassociated sums); FP32 storage/arithmetic to
+<= 2e-6 per kernel application (single precision of the data).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import PolyField, coefficient_matrices, evolve, interior_elements, project, rel_l2
+from seissol_amd import _capi
+from seissol_amd.kernels import (Engine, Local, Neighbor, Spacetime, Time, derivatives_size, global_matrices,
+                                 monomial_integrate, num_basis)
+from seissol_amd.proxy import DeviceLayer, make_setup, run_proxy
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-12, "fp32": 2e-6}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _oracle():
+    oracle.build()
+
+
+def _np(t):
+    return t.detach().cpu().double().numpy()
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("mesh", ["proxy", "periodic"])
+def test_cell_local_matrices_match_oracle(gpu, mesh):
+    setup = make_setup(4, n=777, mesh=mesh, seed=21, dims=(4, 4, 6))
+    print(f"cell-local n={setup.n} mesh={mesh}", flush=True)
+    eng = Engine(4, "fp64")
+    layer = DeviceLayer(eng, setup)
+    _sync()
+    star, ap, am = oracle.cell_local_matrices(setup.vertices, setup.material, setup.neighbors)
+    assert rel_l2(_np(layer.star64), star) < 1e-13
+    assert rel_l2(_np(layer.aplus64), ap) < 1e-12
+    assert rel_l2(_np(layer.aminus64), am) < 1e-12
+
+
+@pytest.mark.parametrize("arith", ["fp64", "fp32"])
+@pytest.mark.parametrize("order", [2, 3, 4, 5, 6, 7])
+def test_ader_with_derivatives(gpu, order, arith):
+    import torch
+    n = 203  # ragged: not a multiple of any block size
+    print(f"ader order={order} arith={arith} n={n}", flush=True)
+    setup = make_setup(order, n=n, mesh="proxy", seed=100 + order)
+    eng = Engine(order, arith)
+    layer = DeviceLayer(eng, setup, with_derivatives=True)
+    coeffs = monomial_integrate(order, 0.0, setup.dt)
+    Spacetime(eng).computeBatchedAder(coeffs, setup.dt, layer.view)
+    _sync()
+    gm = global_matrices(order)["flat"]
+    star = _np(layer.star64)
+    I_ref, d_ref = oracle.ader(order, gm, coeffs, setup.dofs, star, want_derivatives=True)
+    assert rel_l2(_np(layer.time_integrated), I_ref) < TOL[arith]
+    assert rel_l2(_np(layer.derivatives), d_ref) < TOL[arith] * 10
+    # Time::evaluateBatched on the derivatives reproduces the integral
+    out = torch.zeros_like(layer.time_integrated)
+    Time(eng).evaluateBatched(coeffs, layer.derivatives, out, n)
+    _sync()
+    assert rel_l2(_np(out), I_ref) < TOL[arith]
+    # DOFs untouched by ader
+    assert rel_l2(_np(layer.dofs), setup.dofs) < (0 if arith == "fp64" else 1e-7) + 1e-300
+
+
+@pytest.mark.parametrize("arith", ["fp64", "fp32"])
+@pytest.mark.parametrize("order", [3, 4, 6])
+def test_local_integral(gpu, order, arith):
+    import torch
+    n = 150
+    print(f"local order={order} arith={arith}", flush=True)
+    setup = make_setup(order, n=n, mesh="proxy", seed=7 * order)
+    setup.face_types[::5, 2] = _capi.SDG_FACE_DYNAMIC_RUPTURE
+    eng = Engine(order, arith)
+    layer = DeviceLayer(eng, setup)
+    rng = np.random.default_rng(order)
+    I = rng.uniform(-1, 1, setup.dofs.shape)
+    layer.time_integrated.copy_(torch.from_numpy(I).to(layer.time_integrated))
+    Local(eng).computeBatchedIntegral(layer.view)
+    _sync()
+    gm = global_matrices(order)["flat"]
+    I_used = _np(layer.time_integrated)
+    ref = oracle.local(order, gm, I_used, _np(layer.star64), _np(layer.aplus64), setup.dofs, setup.face_types)
+    assert rel_l2(_np(layer.dofs) - setup.dofs, ref - setup.dofs) < TOL[arith] * 10
+
+
+@pytest.mark.parametrize("arith", ["fp64", "fp32"])
+@pytest.mark.parametrize("order", [3, 5, 6, 7])
+def test_neighbor_integral(gpu, order, arith):
+    import torch
+    n = 170
+    print(f"neighbor order={order} arith={arith}", flush=True)
+    setup = make_setup(order, n=n, mesh="proxy", seed=13 * order)
+    setup.face_types[::7, 1] = _capi.SDG_FACE_DYNAMIC_RUPTURE
+    eng = Engine(order, arith)
+    layer = DeviceLayer(eng, setup)
+    rng = np.random.default_rng(order + 1)
+    I = rng.uniform(-1, 1, setup.dofs.shape)
+    layer.time_integrated.copy_(torch.from_numpy(I).to(layer.time_integrated))
+    Neighbor(eng).computeBatchedNeighborsIntegral(layer.view)
+    _sync()
+    gm = global_matrices(order)["flat"]
+    ref = oracle.neighbor(order, gm, _np(layer.time_integrated), setup.neighbors, setup.face_relations,
+                          _np(layer.aminus64), setup.dofs, setup.face_types)
+    assert rel_l2(_np(layer.dofs) - setup.dofs, ref - setup.dofs) < TOL[arith] * 10
+
+
+@pytest.mark.parametrize("arith", ["fp64", "fp32"])
+@pytest.mark.parametrize("order,mesh", [(4, "proxy"), (6, "proxy"), (6, "periodic")])
+def test_proxy_all_steps(gpu, order, mesh, arith):
+    steps = 3
+    setup = make_setup(order, n=1200, mesh=mesh, seed=99, dims=(4, 6, 8))
+    print(f"proxy all order={order} mesh={mesh} arith={arith} n={setup.n}", flush=True)
+    eng = Engine(order, arith)
+    layer = DeviceLayer(eng, setup)
+    run_proxy(eng, layer, "all", steps, setup.dt)
+    _sync()
+    gm = global_matrices(order)["flat"]
+    coeffs = monomial_integrate(order, 0.0, setup.dt)
+    q_ref, I_ref = oracle.proxy_all(order, gm, steps, coeffs, _np(layer.star64), _np(layer.aplus64),
+                                    _np(layer.aminus64), setup.neighbors, setup.face_relations, setup.dofs)
+    err = rel_l2(_np(layer.dofs), q_ref)
+    assert err < (1e-12 if arith == "fp64" else 1e-5), err
+    assert rel_l2(_np(layer.time_integrated), I_ref) < (1e-12 if arith == "fp64" else 1e-5)
+
+
+@pytest.mark.parametrize("kernel", ["local", "neigh", "ader", "localwoader"])
+def test_proxy_kernel_selection(gpu, kernel):
+    order = 4
+    setup = make_setup(order, n=300, mesh="proxy", seed=3)
+    print(f"proxy kernel={kernel}", flush=True)
+    eng = Engine(order, "fp64")
+    layer = DeviceLayer(eng, setup)
+    run_proxy(eng, layer, kernel, 1, setup.dt)
+    _sync()
+    gm = global_matrices(order)["flat"]
+    coeffs = monomial_integrate(order, 0.0, setup.dt)
+    star, ap, am = _np(layer.star64), _np(layer.aplus64), _np(layer.aminus64)
+    I0 = np.zeros_like(setup.dofs)
+    if kernel == "local":
+        I = oracle.ader(order, gm, coeffs, setup.dofs, star)
+        q = oracle.local(order, gm, I, star, ap, setup.dofs)
+    elif kernel == "ader":
+        I = oracle.ader(order, gm, coeffs, setup.dofs, star)
+        q = setup.dofs
+    elif kernel == "neigh":
+        I = I0
+        q = oracle.neighbor(order, gm, I0, setup.neighbors, setup.face_relations, am, setup.dofs)
+    else:
+        I = I0
+        q = oracle.local(order, gm, I0, star, ap, setup.dofs)
+    assert rel_l2(_np(layer.dofs), q) < 1e-12
+    assert np.abs(_np(layer.time_integrated) - I).max() <= 1e-12 * max(1.0, np.abs(I).max())
+
+
+@pytest.mark.parametrize("order", [3, 5, 6])
+def test_gpu_step_exact_for_polynomial_states(gpu, order):
+    import torch
+    dims = (6, 6, 6)
+    setup = make_setup(order, mesh="periodic", dims=dims, seed=3, rho=(2500.0, 2500.0), h=100.0, jitter=0.12)
+    print(f"polynomial exactness order={order} n={setup.n}", flush=True)
+    eng = Engine(order, "fp64")
+    layer = DeviceLayer(eng, setup)
+    A = coefficient_matrices(*setup.material[0])
+    field = PolyField(order, scale=dims[0] * 100.0, rng=np.random.default_rng(3),
+                      amplitude=np.array([1e6] * 6 + [1.0] * 3))
+    q0 = project(order, setup.vertices, field)
+    layer.reset_dofs(q0)
+    run_proxy(eng, layer, "all", 1, setup.dt)
+    _sync()
+    exact = project(order, setup.vertices, evolve(field, A, setup.dt))
+    inner = interior_elements(setup)
+    q1 = _np(layer.dofs)
+    change = np.linalg.norm(exact[inner] - q0[inner])
+    assert np.linalg.norm(q1[inner] - exact[inner]) < 1e-9 * change
+
+
+def test_empty_layer_and_errors(gpu):
+    import ctypes
+    eng = Engine(4, "fp64")
+    empty = _capi.Layer(0, None, None, None, None, None, None, None, None, None)
+    _capi.check(_capi.lib().sdg_compute_batched_ader_local(eng.handle, monomial_integrate(4, 0, 1).ctypes.data,
+                                                            1.0, ctypes.byref(empty), None))
+    _capi.check(_capi.lib().sdg_compute_batched_neighbors_integral(eng.handle, ctypes.byref(empty), None))
+    bad = _capi.Layer(10, None, None, None, None, None, None, None, None, None)
+    assert _capi.lib().sdg_compute_batched_neighbors_integral(eng.handle, ctypes.byref(bad), None) != 0
+    assert b"missing" in _capi.lib().sdg_last_error()
