@@ -66,12 +66,14 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--order", type=int, default=6)
     ap.add_argument("--elements", type=int, default=400_000)
-    ap.add_argument("--form", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--form", choices=["fp64", "fp32", "bf16x3", "bf16x6"], default="bf16x6")
     ap.add_argument("--mesh", choices=["periodic", "proxy"], default="periodic")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--cpu-elements", type=int, default=20_000)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-error-check", action="store_true",
+                    help="skip the rel. L2 comparison of the form against the plain FP64 path")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -133,6 +135,26 @@ def main(argv=None) -> int:
     achieved = ab[dom] * setup.n / (per_kernel[dom] / 1e3) / 1e9
     log(f"per launch: local {per_kernel['local']:.3f} ms, neigh {per_kernel['neigh']:.3f} ms")
 
+    # accuracy of the form: same initial state, same steps, against the plain
+    # FP64 path (itself pinned to the CPU oracle by the parity tests)
+    error = None
+    if args.form != "fp64" and not args.no_error_check:
+        layer.reset_dofs(setup.dofs)
+        run_proxy(eng, layer, "all", args.steps, setup.dt)
+        q_form = layer.dofs.double()
+        del layer
+        eng64 = Engine(args.order, "fp64")
+        layer64 = DeviceLayer(eng64, setup)
+        run_proxy(eng64, layer64, "all", args.steps, setup.dt)
+        torch.cuda.synchronize()
+        diff = (q_form - layer64.dofs).pow(2).sum(dim=(0, 2)).sqrt()
+        ref = layer64.dofs.pow(2).sum(dim=(0, 2)).sqrt()
+        per_q = (diff / ref).tolist()
+        error = {"steps": args.steps, "rel_l2_per_quantity": per_q, "max": max(per_q),
+                 "reference": "plain FP64 GPU path, same input"}
+        log(f"rel L2 vs FP64 after {args.steps} steps: max {max(per_q):.2e}")
+        del layer64, q_form
+
     result = {
         "metric": METRIC,
         "value": value,
@@ -144,12 +166,13 @@ def main(argv=None) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "fp64" else "f32",
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "f32 (2-plane split-bf16 MFMA, f32 accumulate)",
+                  "bf16x6": "f32 (3-plane split-bf16 MFMA, f32 accumulate)"}[args.form],
         "data": "synthetic",
         "config": {"workload": f"elastic ADER-DG element update (proxy `all`), ORDER {args.order}, "
                                f"{setup.n} tetrahedra, {args.mesh} mesh, {args.form}, global time stepping",
                    "order": args.order, "elements": setup.n, "form": args.form, "mesh": args.mesh,
-                   "seed": args.seed, "dt": setup.dt, "finite": finite},
+                   "seed": args.seed, "dt": setup.dt, "finite": finite, "error_vs_fp64": error},
         "roofline": {"bound": "hbm", "kernel": f"proxy `{dom}` ({'ader+volume+local flux' if dom == 'local' else 'neighbour flux'})",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_element": ab[dom],

@@ -29,7 +29,8 @@ extern "C" {
 /* arithmetic of the element-update kernels */
 #define SDG_FP64 0   /* plain FP64: the drop-in for the reference GPU path */
 #define SDG_FP32 1   /* FP32 storage and VALU arithmetic                    */
-#define SDG_BF16X3 2 /* FP32 storage, split-BF16 MFMA products, FP32 accumulate */
+#define SDG_BF16X3 2 /* FP32 storage, MFMA with 2-plane bf16 splits (3 products), FP32 accumulate */
+#define SDG_BF16X6 3 /* FP32 storage, MFMA with 3-plane bf16 splits (6 products), FP32 accumulate */
 
 /* face types (src/Initializer/CellLocalInformation.h FaceType subset) */
 #define SDG_FACE_REGULAR 0
@@ -109,6 +110,9 @@ typedef struct {
   const int32_t* face_neighbors;  /* [n][4], index into time_integrated       */
   const uint8_t* face_relations;  /* [n][4][2]                                */
   const uint8_t* face_types;      /* [n][4] or NULL (all regular)             */
+  void* face_traces;              /* [n][4][9][B2]: rT_f I of every face;     */
+                                  /* required by SDG_BF16X3/X6 (written by the*/
+                                  /* local pass, read by the neighbour pass)  */
 } sdg_layer;
 
 /* Spacetime::computeBatchedAder (src/Kernels/LinearCK/Time.cpp:142-217):

@@ -15,6 +15,7 @@ SDG_OK = 0
 SDG_FP64 = 0
 SDG_FP32 = 1
 SDG_BF16X3 = 2
+SDG_BF16X6 = 3
 SDG_FACE_REGULAR = 0
 SDG_FACE_DYNAMIC_RUPTURE = 1
 
@@ -40,6 +41,7 @@ class Layer(C.Structure):
         ("face_neighbors", P),
         ("face_relations", P),
         ("face_types", P),
+        ("face_traces", P),
     ]
 
 

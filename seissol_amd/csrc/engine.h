@@ -39,6 +39,9 @@ struct EngineState {
   int arithmetic = SDG_FP64;
   void* matrices = nullptr;  // device, real type of the engine
   DeviceMatrixLayout layout{};
+  // matrix-core path (SDG_BF16X3): pre-split bf16 fragments and FP32 tables
+  void* mfmaFrags = nullptr;
+  void* mfmaTables = nullptr;
 };
 
 }  // namespace sdg
@@ -55,4 +58,9 @@ int launchLocal(const EngineState& e, const sdg_layer* layer, hipStream_t stream
 int launchNeighbor(const EngineState& e, const sdg_layer* layer, hipStream_t stream);
 int launchTaylor(const EngineState& e, const double* coeffs, const void* derivatives, void* out, int64_t n,
                  hipStream_t stream);
+// kernels_mfma.hip
+int64_t mfmaFragmentCount(int order);
+int launchLocalMfma(const EngineState& e, const double* coeffs, const sdg_layer* layer, bool writeIntegrated,
+                    hipStream_t stream);
+int launchNeighborMfma(const EngineState& e, const sdg_layer* layer, hipStream_t stream);
 }  // namespace sdg

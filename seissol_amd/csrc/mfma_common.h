@@ -1,0 +1,47 @@
+// Tile geometry shared by the matrix-core kernels and the host code that
+// prepares their pre-split, pre-swizzled matrix fragments.
+#pragma once
+#include <cstdint>
+
+namespace sdg {
+namespace mfma {
+
+constexpr int basis3(int o) { return o * (o + 1) * (o + 2) / 6; }
+constexpr int basis2(int o) { return o * (o + 1) / 2; }
+constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// bf16 elements of one (m-tile, k-step) fragment set: 64 lanes x 8 values for
+// each of the three split planes (hi, mid, lo), stored one plane after the other.
+constexpr int kFragPlane = 512;
+constexpr int kFragPair = 3 * kFragPlane;
+constexpr int kTileElems = 16;        // elements per workgroup tile
+constexpr int kCols = 9 * kTileElems; // GEMM columns (quantity, element)
+
+template <int O>
+struct MShape {
+  static constexpr int B = basis3(O);
+  static constexpr int B2 = basis2(O);
+  static constexpr int MT = cdiv(B, 16);            // row tiles of the basis index
+  static constexpr int KPB = cdiv(B, 32) * 32;      // padded K over the basis index
+  static constexpr int KSB = KPB / 32;
+  static constexpr int FR = 4 * B2;                 // rows of the four face operands
+  static constexpr int MTF = cdiv(FR, 16);
+  static constexpr int KPF = cdiv(FR, 32) * 32;
+  static constexpr int KSF = KPF / 32;
+  static constexpr int KMAX = KPB > KPF ? KPB : KPF;
+  static constexpr int LDX = KMAX + 8;              // bf16 stride of one X column (bank spread)
+  static constexpr int THREADS = 64 * MT;
+  // fragment offsets (bf16 elements)
+  static constexpr int64_t ck = 0;                                   // [3][MT][KSB]
+  static constexpr int64_t vol = ck + 3LL * MT * KSB * kFragPair;    // [3][MT][KSB]
+  static constexpr int64_t fmrt = vol + 3LL * MT * KSB * kFragPair;  // [MTF][KSB]
+  static constexpr int64_t rdm = fmrt + 1LL * MTF * KSB * kFragPair; // [MT][KSF]
+  static constexpr int64_t total = rdm + 1LL * MT * KSF * kFragPair;
+  // float tables: invM2[B2], fP[3][B2][B2]
+  static constexpr int tInvM2 = 0;
+  static constexpr int tFP = B2;
+  static constexpr int tTotal = B2 + 3 * B2 * B2;
+};
+
+}  // namespace mfma
+}  // namespace sdg

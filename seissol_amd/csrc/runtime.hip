@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <functional>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -12,6 +13,7 @@
 #include "engine.h"
 #include "global_matrices.h"
 #include "mesh.h"
+#include "mfma_common.h"
 
 namespace sdg {
 
@@ -65,6 +67,81 @@ int uploadMatrices(EngineState& s, const double* gm) {
     SDG_HIP_CHECK(hipMemcpy(s.matrices, f.data(), bytes, hipMemcpyHostToDevice));
   }
   return SDG_OK;
+}
+
+uint16_t bf16Bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40);  // NaN
+  u += 0x7fffu + ((u >> 16) & 1u);                                                      // round to nearest even
+  return static_cast<uint16_t>(u >> 16);
+}
+float bf16Value(uint16_t h) {
+  const uint32_t u = static_cast<uint32_t>(h) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// Split a row-major FP64 matrix into three bf16 planes (x = hi + mid + lo,
+// each plane the round-to-nearest bf16 of the remainder) laid out in the lane
+// order of v_mfma_f32_16x16x32_bf16's A operand: lane l of fragment (mt, ks)
+// holds A[16 mt + l % 16][32 ks + 8 (l / 16) + i], i < 8.
+void appendFragments(const std::function<double(int, int)>& A, int rows, int cols, int mtiles, int ksteps,
+                     uint16_t* out) {
+  for (int mt = 0; mt < mtiles; ++mt)
+    for (int ks = 0; ks < ksteps; ++ks) {
+      uint16_t* base = out + (static_cast<int64_t>(mt) * ksteps + ks) * mfma::kFragPair;
+      for (int l = 0; l < 64; ++l)
+        for (int i = 0; i < 8; ++i) {
+          const int r = 16 * mt + l % 16, c = 32 * ks + 8 * (l / 16) + i;
+          double rest = (r < rows && c < cols) ? A(r, c) : 0.0;
+          for (int plane = 0; plane < 3; ++plane) {
+            const uint16_t h = bf16Bits(static_cast<float>(rest));
+            base[plane * mfma::kFragPlane + l * 8 + i] = h;
+            rest -= static_cast<double>(bf16Value(h));
+          }
+        }
+    }
+}
+
+template <int O>
+int uploadMfmaData(EngineState& s, const double* gm) {
+  using S = mfma::MShape<O>;
+  const auto g = globalMatrixLayout(O);
+  const int B = S::B, B2 = S::B2;
+  std::vector<uint16_t> frags(S::total, 0);
+  for (int j = 0; j < 3; ++j) {
+    appendFragments([&](int r, int c) { return -gm[g.kDivMT + (j * B + r) * B + c]; }, B, B, S::MT, S::KSB,
+                    frags.data() + S::ck + static_cast<int64_t>(j) * S::MT * S::KSB * mfma::kFragPair);
+    appendFragments([&](int r, int c) { return gm[g.kDivM + (j * B + r) * B + c]; }, B, B, S::MT, S::KSB,
+                    frags.data() + S::vol + static_cast<int64_t>(j) * S::MT * S::KSB * mfma::kFragPair);
+  }
+  appendFragments([&](int r, int c) { return gm[g.fMrT + ((r / B2) * B2 + r % B2) * B + c]; }, S::FR, B, S::MTF,
+                  S::KSB, frags.data() + S::fmrt);
+  appendFragments([&](int r, int c) { return gm[g.rDivM + ((c / B2) * B + r) * B2 + c % B2]; }, B, S::FR, S::MT,
+                  S::KSF, frags.data() + S::rdm);
+  std::vector<float> tables(S::tTotal);
+  for (int m = 0; m < B2; ++m) tables[S::tInvM2 + m] = static_cast<float>(1.0 / gm[g.mass2 + m]);
+  for (int i = 0; i < 3 * B2 * B2; ++i) tables[S::tFP + i] = static_cast<float>(gm[g.fP + i]);
+  if (s.mfmaFrags == nullptr) SDG_HIP_CHECK(hipMalloc(&s.mfmaFrags, frags.size() * sizeof(uint16_t)));
+  if (s.mfmaTables == nullptr) SDG_HIP_CHECK(hipMalloc(&s.mfmaTables, tables.size() * sizeof(float)));
+  SDG_HIP_CHECK(hipMemcpy(s.mfmaFrags, frags.data(), frags.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  SDG_HIP_CHECK(hipMemcpy(s.mfmaTables, tables.data(), tables.size() * sizeof(float), hipMemcpyHostToDevice));
+  return SDG_OK;
+}
+
+int uploadMfma(EngineState& s, const double* gm) {
+  switch (s.order) {
+    case 3: return uploadMfmaData<3>(s, gm);
+    case 4: return uploadMfmaData<4>(s, gm);
+    case 5: return uploadMfmaData<5>(s, gm);
+    case 6: return uploadMfmaData<6>(s, gm);
+    case 7: return uploadMfmaData<7>(s, gm);
+    default:
+      setError("matrix-core arithmetic supports orders 3..7");
+      return SDG_ERR_UNSUPPORTED;
+  }
 }
 
 bool validLayer(const sdg_layer* L, bool needNeighbors) {
@@ -236,7 +313,7 @@ int sdg_engine_create(sdg_engine** engine, int order, int arithmetic) {
     setError("sdg_engine_create: order must be in [2, 7]");
     return SDG_ERR_UNSUPPORTED;
   }
-  if (arithmetic != SDG_FP64 && arithmetic != SDG_FP32 && arithmetic != SDG_BF16X3) {
+  if (arithmetic != SDG_FP64 && arithmetic != SDG_FP32 && arithmetic != SDG_BF16X3 && arithmetic != SDG_BF16X6) {
     setError("sdg_engine_create: unknown arithmetic");
     return SDG_ERR_INVALID;
   }
@@ -245,9 +322,12 @@ int sdg_engine_create(sdg_engine** engine, int order, int arithmetic) {
   e->s.arithmetic = arithmetic;
   std::vector<double> gm(globalMatrixLayout(order).total);
   generateGlobalMatrices(order, gm.data());
-  const int rc = uploadMatrices(e->s, gm.data());
+  int rc = uploadMatrices(e->s, gm.data());
+  if (rc == SDG_OK && (arithmetic == SDG_BF16X3 || arithmetic == SDG_BF16X6)) rc = uploadMfma(e->s, gm.data());
   if (rc != SDG_OK) {
     if (e->s.matrices) (void)hipFree(e->s.matrices);
+    if (e->s.mfmaFrags) (void)hipFree(e->s.mfmaFrags);
+    if (e->s.mfmaTables) (void)hipFree(e->s.mfmaTables);
     delete e;
     return rc;
   }
@@ -257,12 +337,16 @@ int sdg_engine_create(sdg_engine** engine, int order, int arithmetic) {
 
 int sdg_engine_set_global_matrices(sdg_engine* engine, const double* host_matrices) {
   if (engine == nullptr || host_matrices == nullptr) { setError("invalid arguments"); return SDG_ERR_INVALID; }
-  return uploadMatrices(engine->s, host_matrices);
+  const int rc = uploadMatrices(engine->s, host_matrices);
+  if (rc != SDG_OK || (engine->s.arithmetic != SDG_BF16X3 && engine->s.arithmetic != SDG_BF16X6)) return rc;
+  return uploadMfma(engine->s, host_matrices);
 }
 
 int sdg_engine_destroy(sdg_engine* engine) {
   if (engine == nullptr) return SDG_OK;
   if (engine->s.matrices) SDG_HIP_CHECK(hipFree(engine->s.matrices));
+  if (engine->s.mfmaFrags) SDG_HIP_CHECK(hipFree(engine->s.mfmaFrags));
+  if (engine->s.mfmaTables) SDG_HIP_CHECK(hipFree(engine->s.mfmaTables));
   delete engine;
   return SDG_OK;
 }
@@ -281,6 +365,8 @@ int sdg_compute_batched_ader_local(sdg_engine* engine, const double* coeffs, dou
     setError("star/aplus matrices missing");
     return SDG_ERR_INVALID;
   }
+  if ((engine->s.arithmetic == SDG_BF16X3 || engine->s.arithmetic == SDG_BF16X6) && layer->face_traces != nullptr)
+    return launchLocalMfma(engine->s, coeffs, layer, true, static_cast<hipStream_t>(stream));
   return launchAderLocal(engine->s, coeffs, layer, static_cast<hipStream_t>(stream));
 }
 
@@ -306,6 +392,8 @@ int sdg_compute_batched_integral(sdg_engine* engine, const sdg_layer* layer, voi
 
 int sdg_compute_batched_neighbors_integral(sdg_engine* engine, const sdg_layer* layer, void* stream) {
   if (engine == nullptr || !validLayer(layer, true)) return SDG_ERR_INVALID;
+  if ((engine->s.arithmetic == SDG_BF16X3 || engine->s.arithmetic == SDG_BF16X6) && layer->face_traces != nullptr)
+    return launchNeighborMfma(engine->s, layer, static_cast<hipStream_t>(stream));
   return launchNeighbor(engine->s, layer, static_cast<hipStream_t>(stream));
 }
 
@@ -325,8 +413,19 @@ int sdg_proxy_run(sdg_engine* engine, int kernel_id, int steps, double dt, const
   std::vector<double> coeffs(engine->s.order);
   sdg_monomial_integrate(engine->s.order, 0.0, dt, coeffs.data());
   auto st = static_cast<hipStream_t>(stream);
+  const bool mc = engine->s.arithmetic == SDG_BF16X3 || engine->s.arithmetic == SDG_BF16X6;
+  if (mc && layer->num_elements > 0 && layer->face_traces == nullptr && (kernel_id == 0 || kernel_id == 1 || kernel_id == 2)) {
+    setError("sdg_proxy_run: matrix-core arithmetic needs layer->face_traces");
+    return SDG_ERR_INVALID;
+  }
   for (int s = 0; s < steps; ++s) {
     int rc = SDG_OK;
+    if (mc && kernel_id <= 2) {
+      if (kernel_id == 0 || kernel_id == 1) rc = launchLocalMfma(engine->s, coeffs.data(), layer, false, st);
+      if (rc == SDG_OK && (kernel_id == 0 || kernel_id == 2)) rc = launchNeighborMfma(engine->s, layer, st);
+      if (rc != SDG_OK) return rc;
+      continue;
+    }
     switch (kernel_id) {
       case 0:
         rc = launchAderLocal(engine->s, coeffs.data(), layer, st);

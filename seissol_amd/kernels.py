@@ -20,7 +20,8 @@ import numpy as np
 from . import _capi
 from ._capi import Layer, check, lib, ptr
 
-ARITHMETIC = {"fp64": _capi.SDG_FP64, "fp32": _capi.SDG_FP32, "bf16x3": _capi.SDG_BF16X3}
+ARITHMETIC = {"fp64": _capi.SDG_FP64, "fp32": _capi.SDG_FP32, "bf16x3": _capi.SDG_BF16X3,
+              "bf16x6": _capi.SDG_BF16X6}
 
 
 def num_basis(order: int) -> int:

@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _capi
 from ._capi import Layer, check, lib, ptr
-from .kernels import Engine, derivatives_size, num_basis
+from .kernels import Engine, derivatives_size, num_basis, num_face_basis
 
 KERNELS = ("all", "local", "neigh", "ader", "localwoader")
 
@@ -132,9 +132,12 @@ class DeviceLayer:
         self.time_integrated = torch.zeros((n, 9, B), dtype=rt, device=dev)
         self.derivatives = (torch.zeros((n, derivatives_size(setup.order)), dtype=rt, device=dev)
                             if with_derivatives else None)
+        B2 = num_face_basis(setup.order)
+        self.face_traces = (torch.zeros((n, 4, 9, B2), dtype=torch.float32, device=dev)
+                            if engine.arithmetic in ("bf16x3", "bf16x6") else None)
         self.view = Layer(n, ptr(self.dofs), ptr(self.time_integrated), ptr(self.derivatives), ptr(self.star),
                           ptr(self.aplus), ptr(self.aminus), ptr(self.neighbors), ptr(self.face_relations),
-                          ptr(self.face_types))
+                          ptr(self.face_types), ptr(self.face_traces))
 
     def reset_dofs(self, dofs: np.ndarray):
         import torch

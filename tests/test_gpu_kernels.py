@@ -194,3 +194,55 @@ def test_empty_layer_and_errors(gpu):
     bad = _capi.Layer(10, None, None, None, None, None, None, None, None, None)
     assert _capi.lib().sdg_compute_batched_neighbors_integral(eng.handle, ctypes.byref(bad), None) != 0
     assert b"missing" in _capi.lib().sdg_last_error()
+
+
+# ---------------------------------------------------------------------------
+# matrix-core arithmetic (FP32 data, split-BF16 products, FP32 accumulation)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("arith", ["bf16x3", "bf16x6"])
+@pytest.mark.parametrize("order,mesh", [(3, "proxy"), (4, "proxy"), (5, "periodic"), (6, "proxy"), (6, "periodic"),
+                                        (7, "periodic")])
+def test_split_bf16_proxy_all_matches_oracle(gpu, order, mesh, arith):
+    steps = 3
+    setup = make_setup(order, n=900, mesh=mesh, seed=77, dims=(4, 6, 8))
+    setup.face_types[::11, 3] = _capi.SDG_FACE_DYNAMIC_RUPTURE
+    print(f"{arith} proxy all order={order} mesh={mesh} n={setup.n}", flush=True)
+    eng = Engine(order, arith)
+    layer = DeviceLayer(eng, setup)
+    run_proxy(eng, layer, "all", steps, setup.dt)
+    _sync()
+    gm = global_matrices(order)["flat"]
+    coeffs = monomial_integrate(order, 0.0, setup.dt)
+    q_ref, _ = oracle.proxy_all(order, gm, steps, coeffs, _np(layer.star64), _np(layer.aplus64),
+                                _np(layer.aminus64), setup.neighbors, setup.face_relations, setup.dofs,
+                                setup.face_types)
+    err = rel_l2(_np(layer.dofs), q_ref)
+    print(f"{arith} order {order} {mesh}: rel L2 after {steps} steps {err:.2e}")
+    # 2-plane splits: ~2^-17 per contraction; 3-plane splits: FP32 level
+    assert err < (2e-5 if arith == "bf16x3" else 2e-6), err
+
+
+@pytest.mark.parametrize("arith", ["bf16x3", "bf16x6"])
+def test_split_bf16_local_writes_integral_and_traces(gpu, arith):
+    import torch
+    order = 6
+    setup = make_setup(order, n=333, mesh="proxy", seed=5)
+    eng = Engine(order, arith)
+    layer = DeviceLayer(eng, setup)
+    coeffs = monomial_integrate(order, 0.0, setup.dt)
+    Spacetime(eng).computeBatchedAderLocal(coeffs, setup.dt, layer.view)
+    _sync()
+    gm = global_matrices(order)
+    I_ref = oracle.ader(order, gm["flat"], coeffs, setup.dofs, _np(layer.star64))
+    # split-BF16 products: about 2^-17 relative per contraction
+    tol = 1e-5 if arith == "bf16x3" else 1e-6
+    err_i = rel_l2(_np(layer.time_integrated), I_ref)
+    assert err_i < tol, err_i
+    q_ref = oracle.local(order, gm["flat"], I_ref, _np(layer.star64), _np(layer.aplus64), setup.dofs)
+    err_q = rel_l2(_np(layer.dofs) - setup.dofs, q_ref - setup.dofs)
+    assert err_q < 3 * tol, err_q
+    # traces: rT_f I for every face
+    tr_ref = np.einsum("fml,nql->nfqm", gm["rT"], I_ref)
+    err_t = rel_l2(_np(layer.face_traces), tr_ref)
+    assert err_t < tol, err_t
+    print(f"{arith} rel errors: I {err_i:.2e}, dQ {err_q:.2e}, traces {err_t:.2e}")
