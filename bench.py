@@ -59,6 +59,71 @@ def cpu_baseline(order: int, elements: int, steps: int, seed: int) -> dict:
                       f"(CPU restatement, not the reference binary), {sec:.1f} s"}
 
 
+def friction_bench(args) -> dict:
+    """Rate-and-state friction (law 103) at ORDER 4: plain Newton against the
+    MLP warm start, counted in residual evaluations, on the same fault."""
+    import torch
+
+    from seissol_amd._capi import FrictionOptions, ptr
+    from seissol_amd.friction import FaultDriver, FrictionSolver, WarmStartTrainer, make_fault_setup
+
+    steps, dt = args.friction_steps, 1e-3
+    setup = make_fault_setup(num_faces=args.friction_faces, order=4, seed=args.seed)
+    solves = setup.num_points * args.friction_faces * 4 * steps
+    stride = max(1, solves // args.friction_samples)
+    cap = args.friction_samples
+    buf = torch.zeros((cap, 7), dtype=torch.float32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+
+    def run(solver, options=None):
+        drv = FaultDriver(setup, solver, dt)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        drv.advance(steps, options)
+        e.record()
+        torch.cuda.synchronize()
+        return drv, s.elapsed_time(e) / steps
+
+    drv0, ms_plain = run(FrictionSolver(setup.params, 4), FrictionOptions(None, ptr(buf), cap, ptr(cnt), stride, 0))
+    n = min(int(cnt.item()), cap)
+    plain = int(drv0.fault.t["newton_iterations"].sum().item())
+    log(f"friction plain: {plain} evaluations, {ms_plain:.3f} ms/step, {n} samples")
+    g = torch.Generator(device="cuda").manual_seed(args.seed)
+    perm = torch.randperm(n, generator=g, device="cuda")
+    train, held = buf[:n][perm[: n * 19 // 20]].contiguous(), buf[:n][perm[n * 19 // 20:]].contiguous()
+    tr = WarmStartTrainer(seed=args.seed)
+    t0 = time.perf_counter()
+    tr.fit(train, epochs=args.friction_epochs, batch=4096, lr=3e-3)
+    train_s = time.perf_counter() - t0
+    held_err = (tr.predict(held) - held[:, 6]).abs()
+    solver = FrictionSolver(setup.params, 4)
+    solver.mlp_params = tr.params
+    drv1, ms_warm = run(solver)
+    warm = int(drv1.fault.t["newton_iterations"].sum().item())
+    log(f"friction warm: {warm} evaluations, {ms_warm:.3f} ms/step")
+    diffs = {}
+    for name in ("slip_rate_magnitude", "state_variable", "slip1", "traction1"):
+        a, b = drv1.fault.t[name], drv0.fault.t[name]
+        diffs[name] = float((a - b).norm() / b.norm().clamp_min(1e-300))
+    return {
+        "metric": "Newton residual evaluations, FVW rate-and-state ORDER 4, plain vs MLP warm start",
+        "value": plain / warm, "unit": "x fewer evaluations", "higher_is_better": True,
+        "plain_evaluations": plain, "warm_evaluations": warm,
+        "ms_per_dr_step": {"plain": ms_plain, "warm": ms_warm},
+        "config": {"workload": f"planar fault, {args.friction_faces} faces x {setup.num_points} Gauss points, "
+                               f"ORDER 4, {steps} DR steps of {dt} s, nucleation + travelling shear pulse",
+                   "faces": args.friction_faces, "points": setup.num_points, "steps": steps, "seed": args.seed},
+        "training": {"samples": n, "stride": stride, "epochs": args.friction_epochs, "seconds": train_s,
+                     "held_out_median_abs_log10_err": float(held_err.median().item()),
+                     "held_out_p90_abs_log10_err": float(held_err.quantile(0.9).item()),
+                     "creep_estimate_median_abs_log10_err": float(held[:, 6].abs().median().item())},
+        "rel_l2_warm_vs_plain": diffs,
+        "converged": bool(drv1.fault.t["convergence_inner"].all().item()),
+        "dtype": "f64 (Newton and residual), f32 (MLP)", "data": "synthetic",
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +139,12 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error-check", action="store_true",
                     help="skip the rel. L2 comparison of the form against the plain FP64 path")
+    ap.add_argument("--friction", action="store_true",
+                    help="only run the friction benchmark (plain Newton vs MLP warm start)")
+    ap.add_argument("--friction-faces", type=int, default=20_000)
+    ap.add_argument("--friction-steps", type=int, default=60)
+    ap.add_argument("--friction-samples", type=int, default=1_000_000)
+    ap.add_argument("--friction-epochs", type=int, default=6)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -93,6 +164,10 @@ def main(argv=None) -> int:
         return 2
 
     import torch
+
+    if args.friction:
+        print(json.dumps(friction_bench(args)), flush=True)
+        return 0
 
     from seissol_amd.kernels import Engine
     from seissol_amd.proxy import DeviceLayer, make_setup, run_proxy

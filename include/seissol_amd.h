@@ -148,6 +148,117 @@ int sdg_download_reals(sdg_engine* engine, double* host_dst, const void* device_
 int sdg_convert_fp64_to_real(sdg_engine* engine, void* device_dst, const double* device_src, int64_t count,
                              void* stream);
 
+/* ---------------------------------------------------------------------- */
+/* Rate-and-state fast-velocity-weakening friction (friction law 103).      */
+/* ---------------------------------------------------------------------- */
+
+/* Subset of DRParameters (src/Initializer/Parameters/DRParameters.h:55-76). */
+typedef struct {
+  double rs_sr0;                     /* V0                                   */
+  double rs_slip_rate_tolerance;     /* Newton |g| tolerance (1e-8)          */
+  double rs_state_tolerance;         /* fixed-point tolerance (1e-8)         */
+  int32_t rs_max_slip_rate_updates;  /* Newton iterations (60)               */
+  int32_t rs_state_variable_updates; /* fixed-point iterations (10)          */
+  double eta_damp;
+  double eta_damp_end;
+  int32_t nucleation_count;          /* 0 or 1                               */
+  int32_t pad_;
+  double t0;                         /* nucleation rise time                 */
+  double s0;                         /* nucleation start time                */
+} sdg_friction_params;
+
+/* One layer of dynamic-rupture faces in device memory (FP64).  Per-point
+ * arrays are [faces][points] with `points` >= `num_points` real Gauss points
+ * (padding ignored).  Fields follow the friction-law storage of
+ * src/DynamicRupture/FrictionLaws/GpuImpl/FrictionSolverInterface.h:18-90. */
+typedef struct {
+  int64_t num_faces;
+  int32_t points;                    /* stride of per-point arrays           */
+  int32_t num_points;                /* real Gauss points per face (<= 64)   */
+  int32_t order;                     /* time points per step                 */
+  int32_t pad_;
+  const double* imp_and_eta;         /* [f][11] zp zs zpNeig zsNeig etaP etaS invEtaS invZp invZs invZpNeig invZsNeig */
+  const double* q_interpolated_plus; /* [f][order][9][points]                */
+  const double* q_interpolated_minus;
+  double* initial_stress;            /* [f][6][points], nucleation adds to it */
+  double* initial_pressure;          /* [f][points]                          */
+  const double* nucleation_stress;   /* [f][6][points] or NULL               */
+  const double* nucleation_pressure; /* [f][points] or NULL                  */
+  const double* rs_a;                /* [f][points] each                     */
+  const double* rs_sl0;
+  const double* rs_f0;
+  const double* rs_muw;
+  const double* rs_b;
+  const double* rs_srw;
+  double* state_variable;            /* [f][points] each                     */
+  double* slip_rate1;
+  double* slip_rate2;
+  double* slip_rate_magnitude;
+  double* mu;
+  double* traction1;
+  double* traction2;
+  double* slip1;
+  double* slip2;
+  double* accumulated_slip;
+  double* rupture_time;
+  double* peak_slip_rate;
+  double* dyn_stress_time;
+  uint8_t* rupture_time_pending;
+  uint8_t* dyn_stress_time_pending;
+  uint8_t* convergence_inner;
+  uint8_t* convergence_outer;
+  double* imposed_state_plus;        /* [f][9][points]                       */
+  double* imposed_state_minus;
+  int32_t* newton_iterations;        /* optional [f][points]: Newton function evaluations, accumulated */
+} sdg_fault_layer;
+
+/* Options of sdg_friction_evaluate: the warm start of the first slip-rate
+ * solve of each time point by a 6-32-32-1 MLP (FP32 parameters, layout of
+ * sdg_mlp_param_count()), and the recording of (features, target) training
+ * samples from the solves of a run without the MLP. */
+typedef struct {
+  const float* mlp_params;           /* device, or NULL                      */
+  float* samples;                    /* device [capacity][7] or NULL         */
+  int64_t capacity;
+  unsigned long long* sample_count;  /* device counter (atomic)              */
+  int32_t sample_stride;             /* keep 1 of `stride` steady solves     */
+  int32_t pad_;
+} sdg_friction_options;
+
+/* dr::friction_law::FrictionSolver::evaluate for the FVW law on the GPU
+ * (src/DynamicRupture/FrictionLaws/GpuImpl/BaseFrictionSolver.h:253-267,
+ * CPU semantics of CpuImpl/RateAndState.h: face-wide convergence).
+ * delta_t, time_weights: host [order]; resample: device [num_points]^2. */
+int sdg_friction_evaluate(const sdg_friction_params* params, const sdg_fault_layer* layer,
+                          double full_update_time, const double* delta_t, const double* time_weights,
+                          const double* resample, const sdg_friction_options* options, void* stream);
+/* FrictionSolver::computeDeltaT (src/DynamicRupture/FrictionLaws/FrictionSolver.cpp:20-35) */
+int sdg_friction_delta_t(int order, const double* time_points, double* delta_t);
+/* Gauss-Legendre points/weights on [t0, t1] (src/Numerical/Quadrature.h:35-90) */
+int sdg_gauss_legendre(int n, double t0, double t1, double* points, double* weights);
+/* Stroud (collapsed Gauss-Jacobi) rule on the reference triangle with
+ * (order+1)^2 points and the state-variable resampling matrix (the data of
+ * codegen/matrices/dr_stroud_matrices_{order}.json): points [n][2],
+ * weights [n], resample [n][n], n = (order+1)^2. */
+int sdg_dr_quadrature(int order, double* points, double* weights, double* resample);
+/* RateAndStateFastVelocityInitializer::computeInitialStateAndFriction
+ * (src/DynamicRupture/Initializer/RateAndStateInitializer.cpp:154-180), host. */
+int sdg_rs_initial_state(double traction1, double traction2, double pressure, double rs_a, double rs_sr0,
+                         double initial_slip_rate, double* state_variable, double* friction_coefficient);
+
+/* MLP training on the GPU (no counterpart in the reference): parameters
+ * [sdg_mlp_param_count()] FP32 = normalisation (in_mean[6], in_scale[6],
+ * out_mean, out_scale), W1[32][6], b1[32], W2[32][32], b2[32], W3[32], b3. */
+int sdg_mlp_param_count(void);
+/* mean squared error and its gradient over samples [n][7] (features, target),
+ * both in normalised units; grads [param_count] device (overwritten). */
+int sdg_mlp_loss_grad(const float* params, const float* samples, int64_t n, float* grads, float* loss,
+                      void* stream);
+/* one Adam step on params with moments m, v (device, param_count each). */
+int sdg_mlp_adam(float* params, const float* grads, float* m, float* v, int step, float lr, void* stream);
+/* forward pass: out[n] = predicted log10 slip rate for features [n][stride] */
+int sdg_mlp_forward(const float* params, const float* features, int64_t n, int stride, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -118,3 +118,118 @@ def proxy_all(order, gm_flat, steps, coeffs, star, aplus, aminus, neighbors, fac
                            _p(_c(aminus)), _p(_c(neighbors, np.int32)), _p(_c(face_relations, np.uint8)), _p(ft),
                            _p(q), _p(I))
     return q, I
+
+
+# ---------------------------------------------------------------------------
+# friction (oracle/friction_oracle.cpp)
+# ---------------------------------------------------------------------------
+FAULT_FIELDS = ["imp_and_eta", "q_interpolated_plus", "q_interpolated_minus", "initial_stress", "initial_pressure",
+                "nucleation_stress", "nucleation_pressure", "rs_a", "rs_sl0", "rs_f0", "rs_muw", "rs_b", "rs_srw",
+                "state_variable", "slip_rate1", "slip_rate2", "slip_rate_magnitude", "mu", "traction1",
+                "traction2", "slip1", "slip2", "accumulated_slip", "rupture_time", "peak_slip_rate",
+                "dyn_stress_time", "rupture_time_pending", "dyn_stress_time_pending", "convergence_inner",
+                "convergence_outer", "imposed_state_plus", "imposed_state_minus", "newton_iterations"]
+
+
+class OracleFaultLayer(C.Structure):
+    _fields_ = [("num_faces", C.c_int64), ("points", C.c_int32), ("num_points", C.c_int32), ("order", C.c_int32),
+                ("pad_", C.c_int32)] + [(n, C.c_void_p) for n in FAULT_FIELDS]
+
+
+class OracleFrictionParams(C.Structure):
+    _fields_ = [("rs_sr0", C.c_double), ("rs_slip_rate_tolerance", C.c_double), ("rs_state_tolerance", C.c_double),
+                ("rs_max_slip_rate_updates", C.c_int32), ("rs_state_variable_updates", C.c_int32),
+                ("eta_damp", C.c_double), ("eta_damp_end", C.c_double), ("nucleation_count", C.c_int32),
+                ("pad_", C.c_int32), ("t0", C.c_double), ("s0", C.c_double)]
+
+
+def _flib():
+    h = lib()
+    if not hasattr(h, "_friction_ready"):
+        P, D, I = C.c_void_p, C.c_double, C.c_int
+        h.oracle_friction_evaluate.argtypes = [C.POINTER(OracleFaultLayer), C.POINTER(OracleFrictionParams), D, P, P,
+                                               P]
+        h.oracle_friction_evaluate.restype = I
+        for name in ("oracle_arsinhexp", "oracle_arsinhexp_derivative", "oracle_logsinh"):
+            getattr(h, name).argtypes = [D, D]
+            getattr(h, name).restype = D
+        h.oracle_rs_initial_state.argtypes = [D, D, D, D, D, D, P, P]
+        h.oracle_compute_delta_t.argtypes = [I, P, P]
+        h.oracle_mlp_loss_grad.argtypes = [C.c_int64, P, P, P, P]
+        h.oracle_mlp_loss_grad.restype = D
+        h.oracle_precompute_stress.argtypes = [I, I, P, P, P, D, P, P, P]
+        h.oracle_postcompute_imposed.argtypes = [I, I, P, P, P, P, P, P, P, P, P]
+        h._friction_ready = True
+    return h
+
+
+class FaultState:
+    """Host copies of a fault layer (dict of numpy arrays) bound to the
+    oracle's struct; arrays are updated in place by `evaluate`."""
+
+    def __init__(self, arrays: dict, order: int, points: int, num_points: int):
+        self.a = {k: np.ascontiguousarray(v).copy() for k, v in arrays.items()}
+        nf = self.a["rs_a"].shape[0]
+        self.view = OracleFaultLayer(nf, points, num_points, order, 0,
+                                     *[(self.a[n].ctypes.data if n in self.a else None) for n in FAULT_FIELDS])
+
+
+def friction_evaluate(state: FaultState, params: dict, full_update_time, delta_t, time_weights, resample):
+    """One friction step of every face (FrictionSolver::evaluate, law 103)."""
+    prm = OracleFrictionParams(params["rs_sr0"], params["rs_slip_rate_tolerance"], params["rs_state_tolerance"],
+                               params["rs_max_slip_rate_updates"], params["rs_state_variable_updates"],
+                               params["eta_damp"], params["eta_damp_end"], params["nucleation_count"], 0,
+                               params["t0"], params["s0"])
+    dt = _c(delta_t)
+    tw = _c(time_weights)
+    rs = _c(resample)
+    _flib().oracle_friction_evaluate(C.byref(state.view), C.byref(prm), float(full_update_time), _p(dt), _p(tw),
+                                     _p(rs))
+
+
+def arsinhexp(x, c):
+    return _flib().oracle_arsinhexp(float(x), float(c))
+
+
+def arsinhexp_derivative(x, c):
+    return _flib().oracle_arsinhexp_derivative(float(x), float(c))
+
+
+def logsinh(x, c):
+    return _flib().oracle_logsinh(float(x), float(c))
+
+
+def rs_initial_state(t1, t2, pressure, a, sr0, v_ini):
+    s, m = C.c_double(), C.c_double()
+    _flib().oracle_rs_initial_state(t1, t2, pressure, a, sr0, v_ini, C.byref(s), C.byref(m))
+    return s.value, m.value
+
+
+def compute_delta_t(time_points):
+    tp = _c(time_points)
+    out = np.zeros_like(tp)
+    _flib().oracle_compute_delta_t(len(tp), _p(tp), _p(out))
+    return out
+
+
+def mlp_loss_grad(x, t, params):
+    x = _c(x)
+    t = _c(t)
+    params = _c(params)
+    g = np.zeros_like(params)
+    loss = _flib().oracle_mlp_loss_grad(x.shape[0], _p(x), _p(t), _p(params), _p(g))
+    return loss, g
+
+
+def precompute_stress(order, points, imp, q_plus, q_minus, eta_p_damp=1.0):
+    n, t1, t2 = (np.zeros((order, points)) for _ in range(3))
+    _flib().oracle_precompute_stress(order, points, _p(_c(imp)), _p(_c(q_plus)), _p(_c(q_minus)), eta_p_damp,
+                                     _p(n), _p(t1), _p(t2))
+    return n, t1, t2
+
+
+def postcompute_imposed(order, points, imp, normal, t1, t2, q_plus, q_minus, weights):
+    plus, minus = np.zeros((9, points)), np.zeros((9, points))
+    _flib().oracle_postcompute_imposed(order, points, _p(_c(imp)), _p(_c(normal)), _p(_c(t1)), _p(_c(t2)),
+                                       _p(_c(q_plus)), _p(_c(q_minus)), _p(_c(weights)), _p(plus), _p(minus))
+    return plus, minus

@@ -45,6 +45,37 @@ class Layer(C.Structure):
     ]
 
 
+class FrictionParams(C.Structure):
+    """``sdg_friction_params``"""
+
+    _fields_ = [("rs_sr0", D), ("rs_slip_rate_tolerance", D), ("rs_state_tolerance", D),
+                ("rs_max_slip_rate_updates", C.c_int32), ("rs_state_variable_updates", C.c_int32),
+                ("eta_damp", D), ("eta_damp_end", D), ("nucleation_count", C.c_int32), ("pad_", C.c_int32),
+                ("t0", D), ("s0", D)]
+
+
+FAULT_FIELDS = ["imp_and_eta", "q_interpolated_plus", "q_interpolated_minus", "initial_stress", "initial_pressure",
+                "nucleation_stress", "nucleation_pressure", "rs_a", "rs_sl0", "rs_f0", "rs_muw", "rs_b", "rs_srw",
+                "state_variable", "slip_rate1", "slip_rate2", "slip_rate_magnitude", "mu", "traction1",
+                "traction2", "slip1", "slip2", "accumulated_slip", "rupture_time", "peak_slip_rate",
+                "dyn_stress_time", "rupture_time_pending", "dyn_stress_time_pending", "convergence_inner",
+                "convergence_outer", "imposed_state_plus", "imposed_state_minus", "newton_iterations"]
+
+
+class FaultLayer(C.Structure):
+    """``sdg_fault_layer`` (same layout as the oracle's fault layer)."""
+
+    _fields_ = [("num_faces", I64), ("points", C.c_int32), ("num_points", C.c_int32), ("order", C.c_int32),
+                ("pad_", C.c_int32)] + [(name, P) for name in FAULT_FIELDS]
+
+
+class FrictionOptions(C.Structure):
+    """``sdg_friction_options``"""
+
+    _fields_ = [("mlp_params", P), ("samples", P), ("capacity", I64), ("sample_count", P),
+                ("sample_stride", C.c_int32), ("pad_", C.c_int32)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "sdg_last_error": (C.c_char_p, []),
@@ -76,6 +107,16 @@ SIGNATURES = {
     "sdg_upload_reals": (I32, [P, P, P, I64, P]),
     "sdg_download_reals": (I32, [P, P, P, I64, P]),
     "sdg_convert_fp64_to_real": (I32, [P, P, P, I64, P]),
+    "sdg_friction_evaluate": (I32, [C.POINTER(FrictionParams), C.POINTER(FaultLayer), D, P, P, P,
+                                    C.POINTER(FrictionOptions), P]),
+    "sdg_friction_delta_t": (I32, [I32, P, P]),
+    "sdg_gauss_legendre": (I32, [I32, D, D, P, P]),
+    "sdg_dr_quadrature": (I32, [I32, P, P, P]),
+    "sdg_rs_initial_state": (I32, [D, D, D, D, D, D, P, P]),
+    "sdg_mlp_param_count": (I32, []),
+    "sdg_mlp_loss_grad": (I32, [P, P, I64, P, P, P]),
+    "sdg_mlp_adam": (I32, [P, P, P, P, I32, C.c_float, P]),
+    "sdg_mlp_forward": (I32, [P, P, I64, I32, P, P]),
 }
 
 _LIB = None

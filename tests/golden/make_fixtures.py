@@ -11,7 +11,12 @@ reference at run time.  Only numeric data is extracted:
 * ``global_matrix_checksums.json`` -- per-matrix sums / norms of
   codegen/matrices/matrices_{4,10,20,35,56,84}.xml;
 * ``dr_stroud.json`` -- quadrature points, weights and the resampling matrix of
-  codegen/matrices/dr_stroud_matrices_{4,5,6}.json (checksums + order-4 data).
+  codegen/matrices/dr_stroud_matrices_{4,5,6}.json (checksums + order-4 data);
+* ``friction_common.json`` -- the known-answer case of
+  tests/DynamicRupture/FrictionLaws/FrictionSolverCommon.t.h:20-170: inputs
+  (impedances 10/20/15/25, qPlus = o+q+p, qMinus = 2(o+q+p), traction
+  results o+p and 2(o+p), time weights 1..O) and the expected pre- and
+  post-computed stresses, evaluated here at O = 4 time points x 25 points.
 
 Usage: python tests/golden/make_fixtures.py [REFERENCE_ROOT]
 """
@@ -155,12 +160,49 @@ def dr_stroud(ref: Path) -> dict:
     return out
 
 
+def friction_common() -> dict:
+    """Known-answer data of FrictionSolverCommon.t.h (values only)."""
+    O, P = 4, 25
+    zp, zs, zpn, zsn = 10.0, 20.0, 15.0, 25.0
+    etaP, etaS = zp * zpn / (zp + zpn), zs * zsn / (zs + zsn)
+    imp = [zp, zs, zpn, zsn, etaP, etaS, 1 / etaS, 1 / zp, 1 / zs, 1 / zpn, 1 / zsn]
+    o, q, p = np.meshgrid(np.arange(O), np.arange(9), np.arange(P), indexing="ij")
+    qP = (o + q + p).astype(float)
+    qM = 2.0 * (o + q + p)
+    oo, pp = np.meshgrid(np.arange(O), np.arange(P), indexing="ij")
+    t1 = (oo + pp).astype(float)
+    t2 = 2.0 * (oo + pp)
+    w = np.arange(1, O + 1, dtype=float)
+    normal = etaP * (qM[:, 6] - qP[:, 6] + qP[:, 0] / zp + qM[:, 0] / zpn)
+    tr1 = etaS * (qM[:, 7] - qP[:, 7] + qP[:, 3] / zs + qM[:, 3] / zsn)
+    tr2 = etaS * (qM[:, 8] - qP[:, 8] + qP[:, 5] / zs + qM[:, 5] / zsn)
+    # the post-compute subcase runs with zero fault normal stress
+    ns = np.zeros((O, P))
+    minus = np.zeros((9, P))
+    plus = np.zeros((9, P))
+    for k in range(O):
+        minus[0] += w[k] * ns[k]; minus[3] += w[k] * t1[k]; minus[5] += w[k] * t2[k]
+        minus[6] += w[k] * (qM[k, 6] - (ns[k] - qM[k, 0]) / zpn)
+        minus[7] += w[k] * (qM[k, 7] - (t1[k] - qM[k, 3]) / zsn)
+        minus[8] += w[k] * (qM[k, 8] - (t2[k] - qM[k, 5]) / zsn)
+        plus[0] += w[k] * ns[k]; plus[3] += w[k] * t1[k]; plus[5] += w[k] * t2[k]
+        plus[6] += w[k] * (qP[k, 6] + (ns[k] - qP[k, 0]) / zp)
+        plus[7] += w[k] * (qP[k, 7] + (t1[k] - qP[k, 3]) / zs)
+        plus[8] += w[k] * (qP[k, 8] + (t2[k] - qP[k, 5]) / zs)
+    return {"order": O, "points": P, "imp_and_eta": imp, "q_plus": qP.tolist(), "q_minus": qM.tolist(),
+            "traction1": t1.tolist(), "traction2": t2.tolist(), "time_weights": w.tolist(),
+            "expected_normal": normal.tolist(), "expected_traction1": tr1.tolist(),
+            "expected_traction2": tr2.tolist(), "expected_imposed_plus": plus.tolist(),
+            "expected_imposed_minus": minus.tolist(), "epsilon": 1e6 * np.finfo(float).eps}
+
+
 def main(argv):
     ref = Path(argv[1]) if len(argv) > 1 else Path("/root/reference")
     (OUT / "basis_values.json").write_text(json.dumps(basis_values(ref), indent=1))
     (OUT / "godunov_elastic.json").write_text(json.dumps(godunov(ref), indent=1))
     (OUT / "global_matrix_checksums.json").write_text(json.dumps(global_matrix_checksums(ref), indent=1))
     (OUT / "dr_stroud.json").write_text(json.dumps(dr_stroud(ref), indent=1))
+    (OUT / "friction_common.json").write_text(json.dumps(friction_common()))
     print("fixtures written to", OUT)
 
 
