@@ -141,6 +141,19 @@ int sdg_compute_batched_ader_local(sdg_engine* engine, const double* coeffs, dou
 int sdg_proxy_kernel_id(const char* name);
 int sdg_proxy_run(sdg_engine* engine, int kernel_id, int steps, double dt, const sdg_layer* layer, void* stream);
 
+/* Work of one element-update of a proxy kernel (PerformanceEstimate,
+ * src/Monitoring/Metric.h:16-50, summed by src/Proxy/Runner.cpp:120-150),
+ * per element: non-zero and hardware FLOP, bytes (DOF blocks and
+ * per-element operators moved once) and kernel bytes (what the kernels of
+ * this arithmetic move, face traces included). */
+typedef struct {
+  double nonzero_flop;
+  double hardware_flop;
+  double bytes;
+  double kernel_bytes;
+} sdg_work_estimate;
+int sdg_performance_estimate(int order, int arithmetic, int kernel_id, sdg_work_estimate* out);
+
 /* Copy FP64 host data into a layer array of the engine's real type (and
  * back); element counts in reals. */
 int sdg_upload_reals(sdg_engine* engine, void* device_dst, const double* host_src, int64_t count, void* stream);

@@ -69,6 +69,13 @@ class FaultLayer(C.Structure):
                 ("pad_", C.c_int32)] + [(name, P) for name in FAULT_FIELDS]
 
 
+class WorkEstimate(C.Structure):
+    """``sdg_work_estimate``"""
+
+    _fields_ = [("nonzero_flop", C.c_double), ("hardware_flop", C.c_double), ("bytes", C.c_double),
+                ("kernel_bytes", C.c_double)]
+
+
 class FrictionOptions(C.Structure):
     """``sdg_friction_options``"""
 
@@ -117,6 +124,7 @@ SIGNATURES = {
     "sdg_mlp_loss_grad": (I32, [P, P, I64, P, P, P]),
     "sdg_mlp_adam": (I32, [P, P, P, P, I32, C.c_float, P]),
     "sdg_mlp_forward": (I32, [P, P, I64, I32, P, P]),
+    "sdg_performance_estimate": (I32, [I32, I32, I32, C.POINTER(WorkEstimate)]),
 }
 
 _LIB = None

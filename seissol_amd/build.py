@@ -2,7 +2,9 @@
 
 Every source under ``seissol_amd/csrc`` is compiled by ``hipcc`` into an object
 under ``seissol_amd/_build`` (in parallel), then linked into one shared
-library.  Packed FP32 instructions are disabled for all device code.
+library.  Packed FP32 instructions are disabled for all device code.  The
+proxy executable ``seissol_amd/_lib/seissol_amd_proxy`` (host code in
+``seissol_amd/host``) is linked against that library.
 
 Usage: ``python -m seissol_amd.build [-j N]``
 """
@@ -21,6 +23,8 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIBDIR = PKG / "_lib"
 LIBNAME = "libseissol_amd.so"
+PROXY = "seissol_amd_proxy"
+HOST = PKG / "host"
 ARCH = os.environ.get("SDG_OFFLOAD_ARCH", "gfx950")
 
 DEVICE_FLAGS = [
@@ -69,8 +73,17 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, out)
+    # the proxy executable: host code over the C ABI, linked against the library
+    exe = LIBDIR / PROXY
+    exe_tmp = LIBDIR / (PROXY + ".tmp")
+    cmd = [_hipcc(), "-O2", "-std=c++17", "-I", str(PKG.parent / "include"), str(HOST / "proxy.cpp"),
+           "-L", str(LIBDIR), "-lseissol_amd", "-Wl,-rpath,$ORIGIN", "-o", str(exe_tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"proxy build failed\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(exe_tmp, exe)
     if verbose:
-        print(f"built {out}")
+        print(f"built {out} and {exe}")
     return out
 
 

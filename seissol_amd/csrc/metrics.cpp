@@ -1,0 +1,133 @@
+// Per-element work estimates of the proxy kernels (the PerformanceEstimate of
+// src/Monitoring/Metric.h:16-50 that the proxy sums in src/Proxy/Runner.cpp:120-150).
+//
+// Non-zero FLOP: every contraction of the element update counted as 2 flops
+// per non-zero of the sparse operand, from the sparsity of this library's
+// generated global matrices (entries below 1e-13 of the matrix maximum count
+// as zero) and the 24 non-zeros of an elastic star matrix.  Hardware FLOP:
+// the same contractions with dense operands, times the number of bf16
+// products of the split forms.  Bytes: DOF blocks and per-element operators
+// each moved once (global matrices excluded); kernel bytes count the face
+// traces that the matrix-core forms exchange instead of I.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "engine.h"
+#include "global_matrices.h"
+
+namespace sdg {
+namespace {
+
+constexpr int kStarNonZeros = 24;
+
+struct Counts {
+  double nz = 0, dense = 0;
+};
+
+// non-zeros of rows [0, rows) x cols [0, cols) of a row-major block with leading dimension ld
+double nonZeros(const double* m, int rows, int cols, int ld) {
+  double mx = 0;
+  for (int r = 0; r < rows; ++r)
+    for (int c = 0; c < cols; ++c) mx = std::max(mx, std::fabs(m[r * ld + c]));
+  double nz = 0;
+  for (int r = 0; r < rows; ++r)
+    for (int c = 0; c < cols; ++c) nz += std::fabs(m[r * ld + c]) > 1e-13 * mx ? 1 : 0;
+  return nz;
+}
+
+struct Work {
+  Counts ader, volume, localFlux, neighborFlux;
+};
+
+Work elementWork(int order, const std::vector<double>& gm) {
+  const GlobalMatrixLayout L = globalMatrixLayout(order);
+  const int B = numBasis3(order), B2 = numBasis2(order);
+  Work w;
+  // CK: dQ_i[B(O-i)][9] = sum_j kDivMT_j[:B(O-i), :B(O-i+1)] dQ_{i-1} star_j, plus I += c_i dQ_i
+  for (int i = 1; i < order; ++i) {
+    const int rows = numBasis3(order - i), inner = numBasis3(order - i + 1);
+    for (int j = 0; j < 3; ++j) {
+      w.ader.nz += 2.0 * nonZeros(&gm[L.kDivMT + j * B * B], rows, inner, B) * 9 + 2.0 * kStarNonZeros * rows;
+      w.ader.dense += 2.0 * rows * inner * 9 + 2.0 * 81 * rows;
+    }
+  }
+  for (int i = 0; i < order; ++i) {
+    w.ader.nz += 2.0 * numBasis3(order - i) * 9;
+    w.ader.dense += 2.0 * numBasis3(order - i) * 9;
+  }
+  // volume: Q += sum_j kDivM_j (I star_j)
+  for (int j = 0; j < 3; ++j) {
+    w.volume.nz += 2.0 * kStarNonZeros * B + 2.0 * nonZeros(&gm[L.kDivM + j * B * B], B, B, B) * 9;
+    w.volume.dense += 2.0 * 81 * B + 2.0 * B * B * 9;
+  }
+  // local flux of 4 faces: Q += rDivM_f (fMrT_f I) AplusT_f
+  for (int f = 0; f < 4; ++f) {
+    w.localFlux.nz += 2.0 * nonZeros(&gm[L.fMrT + f * B2 * B], B2, B, B) * 9 + 2.0 * 81 * B2 +
+                      2.0 * nonZeros(&gm[L.rDivM + f * B * B2], B, B2, B2) * 9;
+    w.localFlux.dense += 2.0 * B2 * B * 9 + 2.0 * 81 * B2 + 2.0 * B * B2 * 9;
+  }
+  // neighbour flux of 4 faces: Q += rDivM_f fP_h rT_j I_nb AminusT_f, averaged over the
+  // 12 (side j, orientation h) relations
+  double rt = 0, fp = 0, rdm = 0;
+  for (int j = 0; j < 4; ++j) rt += nonZeros(&gm[L.rT + j * B2 * B], B2, B, B) / 4.0;
+  for (int h = 0; h < 3; ++h) fp += nonZeros(&gm[L.fP + h * B2 * B2], B2, B2, B2) / 3.0;
+  for (int f = 0; f < 4; ++f) rdm += nonZeros(&gm[L.rDivM + f * B * B2], B, B2, B2);
+  w.neighborFlux.nz = 4 * (2.0 * rt * 9 + 2.0 * fp * 9 + 2.0 * 81 * B2) + 2.0 * rdm * 9;
+  w.neighborFlux.dense = 4 * (2.0 * B2 * B * 9 + 2.0 * B2 * B2 * 9 + 2.0 * 81 * B2 + 2.0 * B * B2 * 9);
+  return w;
+}
+
+}  // namespace
+}  // namespace sdg
+
+using namespace sdg;
+
+extern "C" int sdg_performance_estimate(int order, int arithmetic, int kernel_id, sdg_work_estimate* out) {
+  if (out == nullptr || order < kMinOrder || order > kMaxOrder || arithmetic < SDG_FP64 ||
+      arithmetic > SDG_BF16X6 || kernel_id < 0 || kernel_id > 4) {
+    setError("sdg_performance_estimate: invalid order, arithmetic, kernel or NULL output");
+    return SDG_ERR_INVALID;
+  }
+  std::vector<double> gm(globalMatrixLayout(order).total);
+  if (generateGlobalMatrices(order, gm.data()) != 0) {
+    setError("sdg_performance_estimate: global matrices");
+    return SDG_ERR_INVALID;
+  }
+  const Work w = elementWork(order, gm);
+  const int B = numBasis3(order), B2 = numBasis2(order);
+  const double s = arithmetic == SDG_FP64 ? 8.0 : 4.0;
+  const double products = arithmetic == SDG_BF16X3 ? 3.0 : (arithmetic == SDG_BF16X6 ? 6.0 : 1.0);
+  const bool traces = arithmetic == SDG_BF16X3 || arithmetic == SDG_BF16X6;
+  const double q = 9.0 * B, traceBytes = 4.0 * 9 * B2 * 4;
+  Counts c;
+  double bytes = 0, kernelBytes = 0;
+  auto add = [&](const Counts& x) {
+    c.nz += x.nz;
+    c.dense += x.dense;
+  };
+  const bool ader = kernel_id == 0 || kernel_id == 1 || kernel_id == 3;
+  const bool local = kernel_id == 0 || kernel_id == 1 || kernel_id == 4;
+  const bool neigh = kernel_id == 0 || kernel_id == 2;
+  if (ader) add(w.ader);
+  if (local) {
+    add(w.volume);
+    add(w.localFlux);
+  }
+  if (neigh) add(w.neighborFlux);
+  // kernel ids: 0 all, 1 local (ader + integral), 2 neigh, 3 ader, 4 localwoader
+  if (kernel_id == 3) bytes = (2 * q + 3 * kStarNonZeros) * s;                    // Q in, I out, star
+  if (kernel_id == 1 || kernel_id == 0) bytes += (3 * q + 3 * kStarNonZeros + 4 * 81) * s;  // Q in/out, I out
+  if (kernel_id == 4) bytes = (3 * q + 3 * kStarNonZeros + 4 * 81) * s;           // I in, Q in/out
+  if (neigh) bytes += (3 * q + 4 * 81) * s;                                        // Q in/out, neighbour I
+  kernelBytes = bytes;
+  // matrix-core forms: the fused local pass writes the face traces rT_f I
+  // instead of I, and the neighbour pass reads the neighbours' traces
+  if (traces && (kernel_id == 0 || kernel_id == 1)) kernelBytes += traceBytes - q * s;
+  if (traces && neigh) kernelBytes += traceBytes - q * s;
+  out->nonzero_flop = c.nz;
+  out->hardware_flop = c.dense * products;
+  out->bytes = bytes;
+  out->kernel_bytes = kernelBytes;
+  return SDG_OK;
+}
