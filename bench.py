@@ -97,6 +97,7 @@ def friction_bench(args) -> dict:
     tr.fit(train, epochs=args.friction_epochs, batch=4096, lr=3e-3)
     train_s = time.perf_counter() - t0
     held_err = (tr.predict(held) - held[:, 6]).abs()
+    off = held[:, 6].abs() > 0.01
     solver = FrictionSolver(setup.params, 4)
     solver.mlp_params = tr.params
     drv1, ms_warm = run(solver)
@@ -117,7 +118,12 @@ def friction_bench(args) -> dict:
         "training": {"samples": n, "stride": stride, "epochs": args.friction_epochs, "seconds": train_s,
                      "held_out_median_abs_log10_err": float(held_err.median().item()),
                      "held_out_p90_abs_log10_err": float(held_err.quantile(0.9).item()),
-                     "creep_estimate_median_abs_log10_err": float(held[:, 6].abs().median().item())},
+                     "creep_estimate_median_abs_log10_err": float(held[:, 6].abs().median().item()),
+                     "where_creep_estimate_off": {
+                         "fraction": float(off.float().mean().item()),
+                         "network_median_abs_log10_err": float(held_err[off].median().item()) if off.any() else None,
+                         "creep_estimate_median_abs_log10_err": float(held[:, 6].abs()[off].median().item())
+                         if off.any() else None}},
         "rel_l2_warm_vs_plain": diffs,
         "converged": bool(drv1.fault.t["convergence_inner"].all().item()),
         "dtype": "f64 (Newton and residual), f32 (MLP)", "data": "synthetic",
@@ -131,7 +137,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--order", type=int, default=6)
     ap.add_argument("--elements", type=int, default=400_000)
-    ap.add_argument("--form", choices=["fp64", "fp32", "bf16x3", "bf16x6"], default="bf16x6")
+    ap.add_argument("--form", choices=["fp64", "fp32", "bf16x3", "bf16x6", "f16x3"], default="bf16x6")
     ap.add_argument("--mesh", choices=["periodic", "proxy"], default="periodic")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--cpu-elements", type=int, default=20_000)
@@ -242,7 +248,8 @@ def main(argv=None) -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "f32 (2-plane split-bf16 MFMA, f32 accumulate)",
-                  "bf16x6": "f32 (3-plane split-bf16 MFMA, f32 accumulate)"}[args.form],
+                  "bf16x6": "f32 (3-plane split-bf16 MFMA, f32 accumulate)",
+                  "f16x3": "f32 (2-plane split-fp16 MFMA on power-of-two scaled operands, f32 accumulate)"}[args.form],
         "data": "synthetic",
         "config": {"workload": f"elastic ADER-DG element update (proxy `all`), ORDER {args.order}, "
                                f"{setup.n} tetrahedra, {args.mesh} mesh, {args.form}, global time stepping",

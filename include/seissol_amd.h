@@ -31,6 +31,8 @@ extern "C" {
 #define SDG_FP32 1   /* FP32 storage and VALU arithmetic                    */
 #define SDG_BF16X3 2 /* FP32 storage, MFMA with 2-plane bf16 splits (3 products), FP32 accumulate */
 #define SDG_BF16X6 3 /* FP32 storage, MFMA with 3-plane bf16 splits (6 products), FP32 accumulate */
+#define SDG_F16X3 4  /* FP32 storage, MFMA with 2-plane fp16 splits of power-of-two scaled operands
+                        (3 products), FP32 accumulate */
 
 /* face types (src/Initializer/CellLocalInformation.h FaceType subset) */
 #define SDG_FACE_REGULAR 0
@@ -111,7 +113,8 @@ typedef struct {
   const uint8_t* face_relations;  /* [n][4][2]                                */
   const uint8_t* face_types;      /* [n][4] or NULL (all regular)             */
   void* face_traces;              /* [n][4][9][B2]: rT_f I of every face;     */
-                                  /* required by SDG_BF16X3/X6 (written by the*/
+                                  /* required by the matrix-core arithmetics  */
+                                  /* SDG_BF16X3/X6, SDG_F16X3 (written by the */
                                   /* local pass, read by the neighbour pass)  */
 } sdg_layer;
 

@@ -16,6 +16,7 @@ SDG_FP64 = 0
 SDG_FP32 = 1
 SDG_BF16X3 = 2
 SDG_BF16X6 = 3
+SDG_F16X3 = 4
 SDG_FACE_REGULAR = 0
 SDG_FACE_DYNAMIC_RUPTURE = 1
 

@@ -34,12 +34,16 @@ struct DeviceMatrixLayout {
 };
 DeviceMatrixLayout deviceMatrixLayout(int order);
 
+inline bool matrixCore(int arithmetic) {
+  return arithmetic == SDG_BF16X3 || arithmetic == SDG_BF16X6 || arithmetic == SDG_F16X3;
+}
+
 struct EngineState {
   int order = 0;
   int arithmetic = SDG_FP64;
   void* matrices = nullptr;  // device, real type of the engine
   DeviceMatrixLayout layout{};
-  // matrix-core path (SDG_BF16X3): pre-split bf16 fragments and FP32 tables
+  // matrix-core arithmetics: pre-split 16-bit fragments and FP32 tables
   void* mfmaFrags = nullptr;
   void* mfmaTables = nullptr;
 };

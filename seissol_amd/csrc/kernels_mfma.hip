@@ -1,19 +1,24 @@
-// Element update on the matrix cores: FP32 data, split-BF16 products, FP32
-// accumulation (arithmetics SDG_BF16X3 and SDG_BF16X6).
+// Element update on the matrix cores: FP32 data, split 16-bit products,
+// FP32 accumulation (arithmetics SDG_BF16X3, SDG_BF16X6 and SDG_F16X3).
 //
 // Every contraction with a global matrix A (kDivMT, kDivM, fMrT, rDivM) is a
 // GEMM  Y(M x 144) += A(M x K) X(K x 144)  over a tile of 16 elements, the 144
-// columns being (quantity p, element e) -> 16 p + e.  It runs as
-// v_mfma_f32_16x16x32_bf16 with both operands split into NS bf16 planes
-// (x = x0 + x1 [+ x2], each plane the rounding of the remainder):
-//   NS = 2: A0X0 + A0X1 + A1X0                      (3 products, ~2^-17)
-//   NS = 3: A0X0 + A0X1 + A1X0 + A0X2 + A2X0 + A1X1  (6 products, ~2^-24)
+// columns being (quantity p, element e) -> 16 p + e.  Both operands are split
+// into planes of a 16-bit type (x = x0 + x1 [+ x2], each plane the rounding of
+// the remainder) and the plane products run as v_mfma_f32_16x16x32_{bf16,f16}:
+//   V = 2  (bf16, 2 planes): A0X0 + A0X1 + A1X0                      ~2^-17
+//   V = 3  (bf16, 3 planes): + A0X2 + A2X0 + A1X1                     ~2^-24
+//   V = 16 (fp16, 2 planes): A0X0 + A0X1 + A1X0                      ~2^-21
+// The fp16 form scales every X column (and every stationary matrix) by a
+// power of two that brings its largest magnitude into [2^13, 2^14), so the
+// 5-bit exponent of fp16 never limits; the scales are exact and are divided
+// out of the FP32 results.
 //
 // One wave owns one 16-row tile of the basis index for all 9 quantities of the
 // 16 elements, so a lane holds rows 4g..4g+3 (g = lane / 16) of element
 // e = lane % 16 for every quantity: the 9x9 star, AplusT and AminusT products
 // that mix quantities are plain register arithmetic.  X lives in LDS
-// column-major ([column][k], k contiguous) as bf16 hi/lo; A is pre-split and
+// column-major ([column][k], k contiguous) as 16-bit planes; A is pre-split and
 // pre-swizzled in global memory so one fragment is one coalesced 16-byte load
 // per lane (L2-resident, shared by all workgroups).
 //
@@ -26,6 +31,8 @@
 // is produced by the neighbour itself, rT_j = diag(1/M2) fMrT_j).
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
+
 #include "engine.h"
 #include "mfma_common.h"
 
@@ -35,7 +42,39 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+template <int V>
+struct Prec;
+template <>
+struct Prec<2> {
+  using T = __bf16;
+  using V8 = bf16x8;
+  using V4 = bf16x4;
+  static constexpr int NX = 2;
+  static constexpr bool scaled = false;
+};
+template <>
+struct Prec<3> {
+  using T = __bf16;
+  using V8 = bf16x8;
+  using V4 = bf16x4;
+  static constexpr int NX = 3;
+  static constexpr bool scaled = false;
+};
+template <>
+struct Prec<16> {
+  using T = _Float16;
+  using V8 = halfx8;
+  using V4 = halfx4;
+  static constexpr int NX = 2;
+  static constexpr bool scaled = true;
+};
+
+// largest |x| of a scaled column: [2^13, 2^14)
+constexpr int kScaleExp = 13;
 
 __device__ constexpr int kStarQ[9][3] = {{6, 7, 8}, {6, 7, 8}, {6, 7, 8}, {6, 7, -1}, {7, 8, -1},
                                          {6, 8, -1}, {0, 3, 5}, {1, 3, 4}, {2, 4, 5}};
@@ -47,104 +86,186 @@ struct CoeffsF {
   float c[O];
 };
 
-template <int NS>
-__device__ inline void split(float x, __bf16 (&parts)[NS]) {
+// geometry of the two kernels per order and variant
+template <int O, int V>
+struct Cfg {
+  using S = MShape<O>;
+  static constexpr int NX = Prec<V>::NX;
+  static constexpr bool scaled = Prec<V>::scaled;
+  static constexpr int LDX_L = S::KMAX + 8;  // local kernel: padded column stride
+  static constexpr int LDS_L = NX * kCols * LDX_L * 2 + kTileElems * 72 * 4 + kTileElems * 81 * 4 +
+                               (scaled ? S::MT * kCols * 4 : 0);
+  static constexpr int WAVES_L = (V != 3 && LDS_L <= 81920) ? 2 : 1;
+  static constexpr int LDX_N0 = S::KMAX + 8;
+  static constexpr int lds_n(int ldx) {
+    return NX * kCols * ldx * 2 + kTileElems * 9 * S::B2 * 4 + kTileElems * 81 * 4 + 3 * S::B2 * S::B2 * 4 +
+           (scaled ? kCols * 4 : 0);
+  }
+  // neighbour kernel: drop the padding when that makes two workgroups fit
+  static constexpr int LDX_N = (V != 3 && lds_n(LDX_N0) > 81920 && lds_n(S::KMAX) <= 81920) ? S::KMAX : LDX_N0;
+  static constexpr int WAVES_N = (V != 3 && lds_n(LDX_N) <= 81920) ? 2 : 1;
+  static constexpr int IPT = cdiv(kTileElems * S::B2, S::THREADS);  // face items per thread
+};
+
+__device__ inline floatx4 mma(bf16x8 a, bf16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ inline floatx4 mma(halfx8 a, halfx8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// split product, smallest terms first
+template <int V>
+__device__ inline floatx4 mfmaSplit(const typename Prec<V>::V8 (&a)[Prec<V>::NX],
+                                    const typename Prec<V>::V8 (&b)[Prec<V>::NX], floatx4 c) {
+  if constexpr (V == 3) {
+    c = mma(a[1], b[1], c);
+    c = mma(a[2], b[0], c);
+    c = mma(a[0], b[2], c);
+  }
+  c = mma(a[1], b[0], c);
+  c = mma(a[0], b[1], c);
+  c = mma(a[0], b[0], c);
+  return c;
+}
+
+template <int V>
+__device__ inline void split(float x, typename Prec<V>::T (&parts)[Prec<V>::NX]) {
+  using T = typename Prec<V>::T;
   float rest = x;
 #pragma unroll
-  for (int i = 0; i < NS; ++i) {
-    parts[i] = static_cast<__bf16>(rest);
+  for (int i = 0; i < Prec<V>::NX; ++i) {
+    parts[i] = static_cast<T>(rest);
     rest -= static_cast<float>(parts[i]);
   }
 }
 
-// X planes: NS consecutive arrays of kCols * LDX bf16
-template <int NS, int LDX>
-__device__ inline void storeX4(__bf16* xs, int col, int k0, const float (&v)[4]) {
-  bf16x4 out[NS];
+// X planes: NX consecutive arrays of kCols * LDX 16-bit values
+template <int V, int LDX>
+__device__ inline void storeX4(typename Prec<V>::T* xs, int col, int k0, const float (&v)[4], float scale) {
+  using P = Prec<V>;
+  typename P::V4 out[P::NX];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    __bf16 parts[NS];
-    split<NS>(v[j], parts);
+    typename P::T parts[P::NX];
+    split<V>(v[j] * scale, parts);
 #pragma unroll
-    for (int i = 0; i < NS; ++i) out[i][j] = parts[i];
+    for (int i = 0; i < P::NX; ++i) out[i][j] = parts[i];
   }
 #pragma unroll
-  for (int i = 0; i < NS; ++i) *reinterpret_cast<bf16x4*>(xs + i * kCols * LDX + col * LDX + k0) = out[i];
+  for (int i = 0; i < P::NX; ++i)
+    *reinterpret_cast<typename P::V4*>(xs + i * kCols * LDX + col * LDX + k0) = out[i];
 }
 
-template <int NS, int LDX>
-__device__ inline void storeX1(__bf16* xs, int col, int k, float v) {
-  __bf16 parts[NS];
-  split<NS>(v, parts);
+template <int V, int LDX>
+__device__ inline void storeX1(typename Prec<V>::T* xs, int col, int k, float v) {
+  typename Prec<V>::T parts[Prec<V>::NX];
+  split<V>(v, parts);
 #pragma unroll
-  for (int i = 0; i < NS; ++i) xs[i * kCols * LDX + col * LDX + k] = parts[i];
+  for (int i = 0; i < Prec<V>::NX; ++i) xs[i * kCols * LDX + col * LDX + k] = parts[i];
 }
 
-template <int NS, int LDX>
-__device__ inline void storeZero4(__bf16* xs, int col, int k0) {
-  bf16x4 z;
+template <int V, int LDX>
+__device__ inline void storeZero4(typename Prec<V>::T* xs, int col, int k0) {
+  using P = Prec<V>;
+  typename P::V4 z;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) z[j] = static_cast<__bf16>(0.0f);
+  for (int j = 0; j < 4; ++j) z[j] = static_cast<typename P::T>(0.0f);
 #pragma unroll
-  for (int i = 0; i < NS; ++i) *reinterpret_cast<bf16x4*>(xs + i * kCols * LDX + col * LDX + k0) = z;
+  for (int i = 0; i < P::NX; ++i) *reinterpret_cast<typename P::V4*>(xs + i * kCols * LDX + col * LDX + k0) = z;
 }
 
-__device__ inline bf16x8 loadFrag(const __bf16* __restrict__ base, int lane) {
-  return *reinterpret_cast<const bf16x8*>(base + lane * 8);
-}
-
-// split product, smallest terms first
-template <int NS>
-__device__ inline floatx4 mfmaSplit(const bf16x8 (&a)[NS], const bf16x8 (&b)[NS], floatx4 c) {
-  if constexpr (NS == 3) {
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], c, 0, 0, 0);
+// power-of-two scale of a column with largest magnitude cm, and its inverse,
+// built from the exponent bits (exponent clamped to [-100, 127])
+__device__ inline void columnScale(float cm, float& scale, float& unscale) {
+  const int biased = static_cast<int>((__float_as_uint(cm) >> 23) & 0xffu);
+  if (cm > 0.0f && biased != 0xff) {
+    const int k = max(biased - 127, -100);
+    scale = __uint_as_float(static_cast<unsigned>(127 + kScaleExp - k) << 23);
+    unscale = __uint_as_float(static_cast<unsigned>(127 - kScaleExp + k) << 23);
+  } else {
+    scale = 1.0f;
+    unscale = 1.0f;
   }
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], c, 0, 0, 0);
-  return c;
+}
+
+// largest magnitude of each quantity column over the rows this wave holds
+// (lanes of one element differ in lane / 16), written to sMax[w][16 q + e]
+template <int R>
+__device__ inline void waveColumnMax(const float (&v)[R][9][4], bool has, float* sMax, int w, int lane) {
+  const int g = lane >> 4, e = lane & 15;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    float m = 0.0f;
+    if (has)
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) m = fmaxf(m, fabsf(v[r][q][j]));
+    m = fmaxf(m, __shfl_xor(m, 16));
+    m = fmaxf(m, __shfl_xor(m, 32));
+    if (g == 0) sMax[w * kCols + q * 16 + e] = m;
+  }
+}
+
+// column scales of this lane's element from the maxima of the WAVES waves
+template <int WAVES>
+__device__ inline void columnScales(const float* sMax, int e, float (&scale)[9], float (&unscale)[9]) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    float cm = 0.0f;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) cm = fmaxf(cm, sMax[w * kCols + q * 16 + e]);
+    columnScale(cm, scale[q], unscale[q]);
+  }
+}
+
+template <int V>
+__device__ inline typename Prec<V>::V8 loadFrag(const uint16_t* __restrict__ base, int lane) {
+  return *reinterpret_cast<const typename Prec<V>::V8*>(base + lane * 8);
 }
 
 // acc[jj][i] += A_jj(tile) X(:, quantity Q0 + i)  for JJ matrices with
 // fragment bases a[jj] and NQ consecutive quantities
-template <int NS, int JJ, int Q0, int NQ, int LDX>
-__device__ inline void gemmTile(floatx4 (&acc)[JJ][NQ], const __bf16* const (&a)[JJ], int ksteps,
-                                const __bf16* xs, int lane) {
+template <int V, int JJ, int Q0, int NQ, int LDX>
+__device__ inline void gemmTile(floatx4 (&acc)[JJ][NQ], const uint16_t* const (&a)[JJ], int ksteps,
+                                const typename Prec<V>::T* xs, int lane) {
+  using P = Prec<V>;
+  constexpr int NX = P::NX;
   const int g = lane >> 4, e = lane & 15;
   for (int kk = 0; kk < ksteps; ++kk) {
-    bf16x8 af[JJ][NS];
+    typename P::V8 af[JJ][NX];
 #pragma unroll
     for (int jj = 0; jj < JJ; ++jj)
 #pragma unroll
-      for (int i = 0; i < NS; ++i) af[jj][i] = loadFrag(a[jj] + kk * kFragPair + i * kFragPlane, lane);
+      for (int i = 0; i < NX; ++i) af[jj][i] = loadFrag<V>(a[jj] + kk * kFragPair + i * kFragPlane, lane);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int off = ((Q0 + q) * 16 + e) * LDX + kk * 32 + 8 * g;
-      bf16x8 bf[NS];
+      typename P::V8 bf[NX];
 #pragma unroll
-      for (int i = 0; i < NS; ++i) bf[i] = *reinterpret_cast<const bf16x8*>(xs + i * kCols * LDX + off);
+      for (int i = 0; i < NX; ++i) bf[i] = *reinterpret_cast<const typename P::V8*>(xs + i * kCols * LDX + off);
 #pragma unroll
-      for (int jj = 0; jj < JJ; ++jj) acc[jj][q] = mfmaSplit<NS>(af[jj], bf, acc[jj][q]);
+      for (int jj = 0; jj < JJ; ++jj) acc[jj][q] = mfmaSplit<V>(af[jj], bf, acc[jj][q]);
     }
   }
 }
 
-// Y += sum_j (A_j X) star_j for the lane's element.  The elastic star matrix
-// maps velocities (6..8) to stresses (0..5) and stresses to velocities, so the
-// contraction runs in two passes of 3 and 6 input quantities, which bounds the
-// live accumulators at 3 x 6 tiles.
-template <int NS, int LDX>
-__device__ inline void gemmStars(const __bf16* const (&a)[3], int ksteps, const __bf16* xs, int lane,
-                                 const float* sS, float (&y)[9][4]) {
+// Y += sum_j (A_j X) star_j for the lane's element, with the matrix and
+// column scales divided out.  The elastic star matrix maps velocities (6..8)
+// to stresses (0..5) and stresses to velocities, so the contraction runs in
+// two passes of 3 and 6 input quantities, which bounds the live accumulators
+// at 3 x 6 tiles.
+template <int V, int LDX>
+__device__ inline void gemmStars(const uint16_t* const (&a)[3], int ksteps, const typename Prec<V>::T* xs, int lane,
+                                 const float* sS, const float (&aUn)[3], const float (&colUn)[9], float (&y)[9][4]) {
   {
     floatx4 acc[3][3];
 #pragma unroll
     for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
       for (int i = 0; i < 3; ++i) acc[jj][i] = floatx4{0, 0, 0, 0};
-    gemmTile<NS, 3, 6, 3, LDX>(acc, a, ksteps, xs, lane);
+    gemmTile<V, 3, 6, 3, LDX>(acc, a, ksteps, xs, lane);
 #pragma unroll
     for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
@@ -152,10 +273,10 @@ __device__ inline void gemmStars(const __bf16* const (&a)[3], int ksteps, const 
 #pragma unroll
         for (int k = 0; k < 3; ++k)
           if (k < kStarCnt[p]) {
-            const float sv = sS[jj * 24 + kStarOff[p] + k];
-            const int qi = kStarQ[p][k] - 6;
+            const int qi = kStarQ[p][k];
+            const float sv = sS[jj * 24 + kStarOff[p] + k] * aUn[jj] * colUn[qi];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) y[p][j] += acc[jj][qi][j] * sv;
+            for (int j = 0; j < 4; ++j) y[p][j] += acc[jj][qi - 6][j] * sv;
           }
   }
   {
@@ -164,32 +285,36 @@ __device__ inline void gemmStars(const __bf16* const (&a)[3], int ksteps, const 
     for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
       for (int i = 0; i < 6; ++i) acc[jj][i] = floatx4{0, 0, 0, 0};
-    gemmTile<NS, 3, 0, 6, LDX>(acc, a, ksteps, xs, lane);
+    gemmTile<V, 3, 0, 6, LDX>(acc, a, ksteps, xs, lane);
 #pragma unroll
     for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
       for (int p = 6; p < 9; ++p)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          const float sv = sS[jj * 24 + kStarOff[p] + k];
           const int qi = kStarQ[p][k];
+          const float sv = sS[jj * 24 + kStarOff[p] + k] * aUn[jj] * colUn[qi];
 #pragma unroll
           for (int j = 0; j < 4; ++j) y[p][j] += acc[jj][qi][j] * sv;
         }
   }
 }
 
-template <int O, int NS>
-__global__ __launch_bounds__(MShape<O>::THREADS, 1) void kLocalMfma(
-    int64_t n, const __bf16* __restrict__ frag, const float* __restrict__ tables, CoeffsF<O> cf,
+template <int O, int V>
+__global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLocalMfma(
+    int64_t n, const uint16_t* __restrict__ frag, const float* __restrict__ tables, CoeffsF<O> cf,
     float* __restrict__ dofs, float* __restrict__ integ, float* __restrict__ traces,
     const float* __restrict__ star, const float* __restrict__ aplus, const uint8_t* __restrict__ faceTypes) {
   using S = MShape<O>;
-  constexpr int B = S::B, B2 = S::B2, MT = S::MT, LDX = S::LDX, FR = S::FR;
+  using C = Cfg<O, V>;
+  using T = typename Prec<V>::T;
+  constexpr int B = S::B, B2 = S::B2, MT = S::MT, LDX = C::LDX_L, FR = S::FR, NX = C::NX;
+  constexpr bool scaled = C::scaled;
   constexpr int TPW = cdiv(S::MTF, MT);
-  __shared__ __bf16 xs[NS * kCols * LDX];
+  __shared__ T xs[NX * kCols * LDX];
   __shared__ float sStar[kTileElems * 72];
   __shared__ float sAp[kTileElems * 81];
+  __shared__ float sMax[scaled ? MT * kCols : 1];
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -197,6 +322,10 @@ __global__ __launch_bounds__(MShape<O>::THREADS, 1) void kLocalMfma(
   const int64_t E0 = static_cast<int64_t>(blockIdx.x) * kTileElems;
   const int64_t elem = E0 + e;
   const bool valid = elem < n;
+  const float* aUn = tables + S::tAUn;
+  const float aUnCk[3] = {aUn[0], aUn[1], aUn[2]};
+  const float aUnVol[3] = {aUn[3], aUn[4], aUn[5]};
+  const float aUnF = aUn[6], aUnR = aUn[7];
 
   for (int i = threadIdx.x; i < kTileElems * 72; i += blockDim.x) {
     const int s = i / 72, r = i % 72, jj = r / 24;
@@ -206,19 +335,29 @@ __global__ __launch_bounds__(MShape<O>::THREADS, 1) void kLocalMfma(
     sStar[i] = (E0 + s < n) ? star[(E0 + s) * 243 + 81 * jj + q + 9 * p] : 0.0f;
   }
 
-  float q[9][4], iacc[9][4];
+  // Q itself is read again at the end, so only the time integral stays live
+  float q[1][9][4], iacc[1][9][4];
 #pragma unroll
   for (int p = 0; p < 9; ++p)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int row = 16 * w + 4 * g + j;
-      q[p][j] = (valid && row < B) ? dofs[elem * 9 * B + p * B + row] : 0.0f;
-      iacc[p][j] = cf.c[0] * q[p][j];
+      q[0][p][j] = (valid && row < B) ? dofs[elem * 9 * B + p * B + row] : 0.0f;
+      iacc[0][p][j] = cf.c[0] * q[0][p][j];
     }
+  // X <- Q
+  float cs[9], un[9];
 #pragma unroll
-  for (int p = 0; p < 9; ++p) storeX4<NS, LDX>(xs, p * 16 + e, 16 * w + 4 * g, q[p]);
+  for (int p = 0; p < 9; ++p) cs[p] = un[p] = 1.0f;
+  if constexpr (scaled) {
+    waveColumnMax<1>(q, true, sMax, w, lane);
+    __syncthreads();
+    columnScales<MT>(sMax, e, cs, un);
+  }
+#pragma unroll
+  for (int p = 0; p < 9; ++p) storeX4<V, LDX>(xs, p * 16 + e, 16 * w + 4 * g, q[0][p], cs[p]);
   for (int t = MT + w; t < S::KPB / 16; t += MT)
-    for (int p = 0; p < 9; ++p) storeZero4<NS, LDX>(xs, p * 16 + e, 16 * t + 4 * g);
+    for (int p = 0; p < 9; ++p) storeZero4<V, LDX>(xs, p * 16 + e, 16 * t + 4 * g);
   __syncthreads();
   const float* sS = sStar + e * 72;
 
@@ -227,34 +366,36 @@ __global__ __launch_bounds__(MShape<O>::THREADS, 1) void kLocalMfma(
   for (int d = 1; d < O; ++d) {
     const int m = basis3(O - d), nIn = basis3(O - d + 1);
     const int mtiles = cdiv(m, 16), ks = cdiv(nIn, 32), kpNext = cdiv(m, 32) * 32;
-    float dq[9][4];
+    float dq[1][9][4];
 #pragma unroll
     for (int p = 0; p < 9; ++p)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) dq[p][j] = 0.0f;
+      for (int j = 0; j < 4; ++j) dq[0][p][j] = 0.0f;
     if (w < mtiles) {
-      const __bf16* a[3] = {frag + S::ck + (0LL * MT + w) * S::KSB * kFragPair,
-                            frag + S::ck + (1LL * MT + w) * S::KSB * kFragPair,
-                            frag + S::ck + (2LL * MT + w) * S::KSB * kFragPair};
-      gemmStars<NS, LDX>(a, ks, xs, lane, sS, dq);
+      const uint16_t* a[3] = {frag + S::ck + (0LL * MT + w) * S::KSB * kFragPair,
+                              frag + S::ck + (1LL * MT + w) * S::KSB * kFragPair,
+                              frag + S::ck + (2LL * MT + w) * S::KSB * kFragPair};
+      gemmStars<V, LDX>(a, ks, xs, lane, sS, aUnCk, un, dq[0]);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         if (16 * w + 4 * g + j >= m)
 #pragma unroll
-          for (int p = 0; p < 9; ++p) dq[p][j] = 0.0f;
+          for (int p = 0; p < 9; ++p) dq[0][p][j] = 0.0f;
     }
+    if constexpr (scaled) waveColumnMax<1>(dq, w < mtiles, sMax, w, lane);
     __syncthreads();
+    if constexpr (scaled) columnScales<MT>(sMax, e, cs, un);
     if (w < mtiles) {
 #pragma unroll
       for (int p = 0; p < 9; ++p) {
-        storeX4<NS, LDX>(xs, p * 16 + e, 16 * w + 4 * g, dq[p]);
+        storeX4<V, LDX>(xs, p * 16 + e, 16 * w + 4 * g, dq[0][p], cs[p]);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) iacc[p][j] += cf.c[d] * dq[p][j];
+        for (int j = 0; j < 4; ++j) iacc[0][p][j] += cf.c[d] * dq[0][p][j];
       }
     }
     for (int t = w; t < kpNext / 16; t += MT)
       if (t >= mtiles)
-        for (int p = 0; p < 9; ++p) storeZero4<NS, LDX>(xs, p * 16 + e, 16 * t + 4 * g);
+        for (int p = 0; p < 9; ++p) storeZero4<V, LDX>(xs, p * 16 + e, 16 * t + 4 * g);
     __syncthreads();
   }
 
@@ -265,21 +406,31 @@ __global__ __launch_bounds__(MShape<O>::THREADS, 1) void kLocalMfma(
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int row = 16 * w + 4 * g + j;
-        if (row < B) integ[elem * 9 * B + p * B + row] = iacc[p][j];
+        if (row < B) integ[elem * 9 * B + p * B + row] = iacc[0][p][j];
       }
   }
+  if constexpr (scaled) {
+    waveColumnMax<1>(iacc, true, sMax, w, lane);
+    __syncthreads();
+    columnScales<MT>(sMax, e, cs, un);
+  }
 #pragma unroll
-  for (int p = 0; p < 9; ++p) storeX4<NS, LDX>(xs, p * 16 + e, 16 * w + 4 * g, iacc[p]);
+  for (int p = 0; p < 9; ++p) storeX4<V, LDX>(xs, p * 16 + e, 16 * w + 4 * g, iacc[0][p], cs[p]);
   for (int t = MT + w; t < S::KPB / 16; t += MT)
-    for (int p = 0; p < 9; ++p) storeZero4<NS, LDX>(xs, p * 16 + e, 16 * t + 4 * g);
+    for (int p = 0; p < 9; ++p) storeZero4<V, LDX>(xs, p * 16 + e, 16 * t + 4 * g);
   __syncthreads();
 
   // volume: Q += sum_j kDivM_j I star_j
+  float vol[9][4];
+#pragma unroll
+  for (int p = 0; p < 9; ++p)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) vol[p][j] = 0.0f;
   {
-    const __bf16* a[3] = {frag + S::vol + (0LL * MT + w) * S::KSB * kFragPair,
-                          frag + S::vol + (1LL * MT + w) * S::KSB * kFragPair,
-                          frag + S::vol + (2LL * MT + w) * S::KSB * kFragPair};
-    gemmStars<NS, LDX>(a, S::KSB, xs, lane, sS, q);
+    const uint16_t* a[3] = {frag + S::vol + (0LL * MT + w) * S::KSB * kFragPair,
+                            frag + S::vol + (1LL * MT + w) * S::KSB * kFragPair,
+                            frag + S::vol + (2LL * MT + w) * S::KSB * kFragPair};
+    gemmStars<V, LDX>(a, S::KSB, xs, lane, sS, aUnVol, un, vol);
   }
 
   // face traces T = fMrT_f I (rows f * B2 + m)
@@ -295,12 +446,12 @@ __global__ __launch_bounds__(MShape<O>::THREADS, 1) void kLocalMfma(
       floatx4 acc[1][9];
 #pragma unroll
       for (int p = 0; p < 9; ++p) acc[0][p] = floatx4{0, 0, 0, 0};
-      const __bf16* a[1] = {frag + S::fmrt + static_cast<int64_t>(t) * S::KSB * kFragPair};
-      gemmTile<NS, 1, 0, 9, LDX>(acc, a, S::KSB, xs, lane);
+      const uint16_t* a[1] = {frag + S::fmrt + static_cast<int64_t>(t) * S::KSB * kFragPair};
+      gemmTile<V, 1, 0, 9, LDX>(acc, a, S::KSB, xs, lane);
 #pragma unroll
       for (int p = 0; p < 9; ++p)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) tr[ti][p][j] = acc[0][p][j];
+        for (int j = 0; j < 4; ++j) tr[ti][p][j] = acc[0][p][j] * (aUnF * un[p]);
       // neighbour-side traces rT_f I = T / M2
       if (valid) {
 #pragma unroll
@@ -317,14 +468,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, 1) void kLocalMfma(
     }
   }
 
-  // W = T AplusT_f, face by face (AplusT staged in LDS)
-  float wv[TPW][9][4];
-#pragma unroll
-  for (int ti = 0; ti < TPW; ++ti)
-#pragma unroll
-    for (int p = 0; p < 9; ++p)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) wv[ti][p][j] = 0.0f;
+  // W = T AplusT_f in place of T, face by face (AplusT staged in LDS)
 #pragma unroll 1
   for (int f = 0; f < 4; ++f) {
     __syncthreads();
@@ -341,28 +485,33 @@ __global__ __launch_bounds__(MShape<O>::THREADS, 1) void kLocalMfma(
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int r = 16 * t + 4 * g + j;
-        if (t < S::MTF && r < FR && r / B2 == f && !dr) {
+        if (t < S::MTF && r < FR && r / B2 == f) {
+          float wrow[9];
 #pragma unroll
           for (int p = 0; p < 9; ++p) {
             float acc = 0.0f;
 #pragma unroll
             for (int qq = 0; qq < 9; ++qq) acc += tr[ti][qq][j] * A[qq + 9 * p];
-            wv[ti][p][j] = acc;
+            wrow[p] = dr ? 0.0f : acc;
           }
+#pragma unroll
+          for (int p = 0; p < 9; ++p) tr[ti][p][j] = wrow[p];
         }
       }
     }
   }
+  if constexpr (scaled) waveColumnMax<TPW>(tr, true, sMax, w, lane);
   __syncthreads();
+  if constexpr (scaled) columnScales<MT>(sMax, e, cs, un);
 #pragma unroll
   for (int ti = 0; ti < TPW; ++ti) {
     const int t = w + ti * MT;
     if (t < S::MTF)
 #pragma unroll
-      for (int p = 0; p < 9; ++p) storeX4<NS, LDX>(xs, p * 16 + e, 16 * t + 4 * g, wv[ti][p]);
+      for (int p = 0; p < 9; ++p) storeX4<V, LDX>(xs, p * 16 + e, 16 * t + 4 * g, tr[ti][p], cs[p]);
   }
   for (int t = S::MTF + w; t < S::KPF / 16; t += MT)
-    for (int p = 0; p < 9; ++p) storeZero4<NS, LDX>(xs, p * 16 + e, 16 * t + 4 * g);
+    for (int p = 0; p < 9; ++p) storeZero4<V, LDX>(xs, p * 16 + e, 16 * t + 4 * g);
   __syncthreads();
 
   // local flux: Q += [rDivM_0 .. rDivM_3] W
@@ -370,31 +519,39 @@ __global__ __launch_bounds__(MShape<O>::THREADS, 1) void kLocalMfma(
     floatx4 acc[1][9];
 #pragma unroll
     for (int p = 0; p < 9; ++p) acc[0][p] = floatx4{0, 0, 0, 0};
-    const __bf16* a[1] = {frag + S::rdm + static_cast<int64_t>(w) * S::KSF * kFragPair};
-    gemmTile<NS, 1, 0, 9, LDX>(acc, a, S::KSF, xs, lane);
+    const uint16_t* a[1] = {frag + S::rdm + static_cast<int64_t>(w) * S::KSF * kFragPair};
+    gemmTile<V, 1, 0, 9, LDX>(acc, a, S::KSF, xs, lane);
     if (valid) {
 #pragma unroll
       for (int p = 0; p < 9; ++p)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int row = 16 * w + 4 * g + j;
-          if (row < B) dofs[elem * 9 * B + p * B + row] = q[p][j] + acc[0][p][j];
+          if (row < B) {
+            float* d = dofs + elem * 9 * B + p * B + row;
+            *d = *d + (vol[p][j] + acc[0][p][j] * (aUnR * un[p]));
+          }
         }
     }
   }
 }
 
-template <int O, int NS>
-__global__ __launch_bounds__(MShape<O>::THREADS, 1) void kNeighborMfma(
-    int64_t n, const __bf16* __restrict__ frag, const float* __restrict__ tables, float* __restrict__ dofs,
+template <int O, int V>
+__global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNeighborMfma(
+    int64_t n, const uint16_t* __restrict__ frag, const float* __restrict__ tables, float* __restrict__ dofs,
     const float* __restrict__ traces, const int32_t* __restrict__ neighbors, const uint8_t* __restrict__ faceRel,
     const float* __restrict__ aminus, const uint8_t* __restrict__ faceTypes) {
   using S = MShape<O>;
-  constexpr int B = S::B, B2 = S::B2, MT = S::MT, LDX = S::LDX, FR = S::FR;
-  __shared__ __bf16 xs[NS * kCols * LDX];
+  using C = Cfg<O, V>;
+  using T = typename Prec<V>::T;
+  constexpr int B = S::B, B2 = S::B2, LDX = C::LDX_N, FR = S::FR, NX = C::NX, IPT = C::IPT;
+  constexpr bool scaled = C::scaled;
+  constexpr int ITEMS = kTileElems * B2;
+  __shared__ T xs[NX * kCols * LDX];
   __shared__ float sTr[kTileElems * 9 * B2];
   __shared__ float sAm[kTileElems * 81];
   __shared__ float sFP[3 * B2 * B2];
+  __shared__ unsigned sMaxU[scaled ? kCols : 1];
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -404,13 +561,17 @@ __global__ __launch_bounds__(MShape<O>::THREADS, 1) void kNeighborMfma(
   const bool valid = elem < n;
 
   for (int i = threadIdx.x; i < 3 * B2 * B2; i += blockDim.x) sFP[i] = tables[S::tFP + i];
+  if constexpr (scaled)
+    for (int i = threadIdx.x; i < kCols; i += blockDim.x) sMaxU[i] = 0u;
   // zero padding rows of the face operand
   for (int i = threadIdx.x; i < kCols * (S::KPF - FR); i += blockDim.x) {
     const int col = i / (S::KPF - FR), k = FR + i % (S::KPF - FR);
 #pragma unroll
-    for (int pl = 0; pl < NS; ++pl) xs[pl * kCols * LDX + col * LDX + k] = static_cast<__bf16>(0.0f);
+    for (int pl = 0; pl < NX; ++pl) xs[pl * kCols * LDX + col * LDX + k] = static_cast<T>(0.0f);
   }
-#pragma unroll 1
+  // the fp16 form keeps the face operand in registers until its column scales are known
+  float wv[scaled ? 4 : 1][scaled ? IPT : 1][9];
+#pragma unroll
   for (int f = 0; f < 4; ++f) {
     __syncthreads();
     for (int i = threadIdx.x; i < kTileElems * 9 * B2; i += blockDim.x) {
@@ -429,50 +590,86 @@ __global__ __launch_bounds__(MShape<O>::THREADS, 1) void kNeighborMfma(
       sAm[i] = (E0 + s < n) ? aminus[(E0 + s) * 324 + f * 81 + r] : 0.0f;
     }
     __syncthreads();
-    for (int item = threadIdx.x; item < kTileElems * B2; item += blockDim.x) {
-      const int s = item / B2, row = item % B2;
-      const int64_t el = E0 + s;
-      const bool regular = el < n && (faceTypes == nullptr || faceTypes[el * 4 + f] == SDG_FACE_REGULAR);
-      const int h = regular ? min(static_cast<int>(faceRel[(el * 4 + f) * 2 + 1]), 2) : 0;
-      float u[9];
 #pragma unroll
-      for (int qq = 0; qq < 9; ++qq) u[qq] = 0.0f;
-      const float* fp = sFP + (h * B2 + row) * B2;
-      const float* tr = sTr + s * 9 * B2;
-      for (int mm = 0; mm < B2; ++mm) {
-        const float c = fp[mm];
+    for (int it = 0; it < IPT; ++it) {
+      const int item = threadIdx.x + it * S::THREADS;
+      if (item < ITEMS) {
+        const int s = item / B2, row = item % B2;
+        const int64_t el = E0 + s;
+        const bool regular = el < n && (faceTypes == nullptr || faceTypes[el * 4 + f] == SDG_FACE_REGULAR);
+        const int h = regular ? min(static_cast<int>(faceRel[(el * 4 + f) * 2 + 1]), 2) : 0;
+        float u[9];
 #pragma unroll
-        for (int qq = 0; qq < 9; ++qq) u[qq] += c * tr[qq * B2 + mm];
-      }
-      const float* A = sAm + s * 81;
+        for (int qq = 0; qq < 9; ++qq) u[qq] = 0.0f;
+        const float* fp = sFP + (h * B2 + row) * B2;
+        const float* tr = sTr + s * 9 * B2;
+        for (int mm = 0; mm < B2; ++mm) {
+          const float c = fp[mm];
 #pragma unroll
-      for (int p = 0; p < 9; ++p) {
-        float acc = 0.0f;
-        if (regular)
+          for (int qq = 0; qq < 9; ++qq) u[qq] += c * tr[qq * B2 + mm];
+        }
+        const float* A = sAm + s * 81;
 #pragma unroll
-          for (int qq = 0; qq < 9; ++qq) acc += u[qq] * A[qq + 9 * p];
-        storeX1<NS, LDX>(xs, p * 16 + s, f * B2 + row, acc);
+        for (int p = 0; p < 9; ++p) {
+          float acc = 0.0f;
+          if (regular)
+#pragma unroll
+            for (int qq = 0; qq < 9; ++qq) acc += u[qq] * A[qq + 9 * p];
+          if constexpr (scaled) {
+            wv[f][it][p] = acc;
+            atomicMax(&sMaxU[p * 16 + s], __float_as_uint(fabsf(acc)));
+          } else {
+            storeX1<V, LDX>(xs, p * 16 + s, f * B2 + row, acc);
+          }
+        }
       }
     }
   }
   __syncthreads();
+  float un[9];
+#pragma unroll
+  for (int p = 0; p < 9; ++p) un[p] = 1.0f;
+  if constexpr (scaled) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+      for (int it = 0; it < IPT; ++it) {
+        const int item = threadIdx.x + it * S::THREADS;
+        if (item < ITEMS) {
+          const int s = item / B2, row = item % B2;
+#pragma unroll
+          for (int p = 0; p < 9; ++p) {
+            float sc, inv;
+            columnScale(__uint_as_float(sMaxU[p * 16 + s]), sc, inv);
+            storeX1<V, LDX>(xs, p * 16 + s, f * B2 + row, wv[f][it][p] * sc);
+          }
+        }
+      }
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+      float sc;
+      columnScale(__uint_as_float(sMaxU[p * 16 + e]), sc, un[p]);
+    }
+    __syncthreads();
+  }
   floatx4 acc[1][9];
 #pragma unroll
   for (int p = 0; p < 9; ++p) acc[0][p] = floatx4{0, 0, 0, 0};
-  const __bf16* a[1] = {frag + S::rdm + static_cast<int64_t>(w) * S::KSF * kFragPair};
-  gemmTile<NS, 1, 0, 9, LDX>(acc, a, S::KSF, xs, lane);
+  const uint16_t* a[1] = {frag + S::rdm + static_cast<int64_t>(w) * S::KSF * kFragPair};
+  gemmTile<V, 1, 0, 9, LDX>(acc, a, S::KSF, xs, lane);
+  const float aUnR = tables[S::tAUn + 7];
   if (valid) {
 #pragma unroll
     for (int p = 0; p < 9; ++p)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int row = 16 * w + 4 * g + j;
-        if (row < B) dofs[elem * 9 * B + p * B + row] += acc[0][p][j];
+        if (row < B) dofs[elem * 9 * B + p * B + row] += acc[0][p][j] * (aUnR * un[p]);
       }
   }
 }
 
-template <int O, int NS>
+template <int O, int V>
 int localImpl(const EngineState& st, const double* coeffs, const sdg_layer* L, bool writeIntegrated,
               hipStream_t stream) {
   using S = MShape<O>;
@@ -481,26 +678,46 @@ int localImpl(const EngineState& st, const double* coeffs, const sdg_layer* L, b
   CoeffsF<O> cf{};
   for (int i = 0; i < O; ++i) cf.c[i] = static_cast<float>(coeffs[i]);
   const dim3 grid(static_cast<unsigned>((n + kTileElems - 1) / kTileElems));
-  kLocalMfma<O, NS><<<grid, S::THREADS, 0, stream>>>(
-      n, static_cast<const __bf16*>(st.mfmaFrags), static_cast<const float*>(st.mfmaTables), cf,
-      static_cast<float*>(L->dofs), writeIntegrated ? static_cast<float*>(L->time_integrated) : nullptr, static_cast<float*>(L->face_traces),
-      static_cast<const float*>(L->star), static_cast<const float*>(L->aplus), L->face_types);
+  kLocalMfma<O, V><<<grid, S::THREADS, 0, stream>>>(
+      n, static_cast<const uint16_t*>(st.mfmaFrags), static_cast<const float*>(st.mfmaTables), cf,
+      static_cast<float*>(L->dofs), writeIntegrated ? static_cast<float*>(L->time_integrated) : nullptr,
+      static_cast<float*>(L->face_traces), static_cast<const float*>(L->star), static_cast<const float*>(L->aplus),
+      L->face_types);
   SDG_HIP_CHECK(hipGetLastError());
   return SDG_OK;
 }
 
-template <int O, int NS>
+template <int O, int V>
 int neighborImpl(const EngineState& st, const sdg_layer* L, hipStream_t stream) {
   using S = MShape<O>;
   const int64_t n = L->num_elements;
   if (n == 0) return SDG_OK;
   const dim3 grid(static_cast<unsigned>((n + kTileElems - 1) / kTileElems));
-  kNeighborMfma<O, NS><<<grid, S::THREADS, 0, stream>>>(
-      n, static_cast<const __bf16*>(st.mfmaFrags), static_cast<const float*>(st.mfmaTables),
+  kNeighborMfma<O, V><<<grid, S::THREADS, 0, stream>>>(
+      n, static_cast<const uint16_t*>(st.mfmaFrags), static_cast<const float*>(st.mfmaTables),
       static_cast<float*>(L->dofs), static_cast<const float*>(L->face_traces), L->face_neighbors, L->face_relations,
       static_cast<const float*>(L->aminus), L->face_types);
   SDG_HIP_CHECK(hipGetLastError());
   return SDG_OK;
+}
+
+template <int O>
+int localDispatch(const EngineState& e, const double* coeffs, const sdg_layer* layer, bool writeIntegrated,
+                  hipStream_t stream) {
+  switch (e.arithmetic) {
+    case SDG_BF16X6: return localImpl<O, 3>(e, coeffs, layer, writeIntegrated, stream);
+    case SDG_F16X3: return localImpl<O, 16>(e, coeffs, layer, writeIntegrated, stream);
+    default: return localImpl<O, 2>(e, coeffs, layer, writeIntegrated, stream);
+  }
+}
+
+template <int O>
+int neighborDispatch(const EngineState& e, const sdg_layer* layer, hipStream_t stream) {
+  switch (e.arithmetic) {
+    case SDG_BF16X6: return neighborImpl<O, 3>(e, layer, stream);
+    case SDG_F16X3: return neighborImpl<O, 16>(e, layer, stream);
+    default: return neighborImpl<O, 2>(e, layer, stream);
+  }
 }
 
 }  // namespace
@@ -520,32 +737,22 @@ int64_t mfmaFragmentCount(int order) {
 int launchLocalMfma(const EngineState& e, const double* coeffs, const sdg_layer* layer, bool writeIntegrated,
                     hipStream_t stream) {
   switch (e.order) {
-    case 3: return e.arithmetic == SDG_BF16X6 ? mfma::localImpl<3, 3>(e, coeffs, layer, writeIntegrated, stream)
-                                          : mfma::localImpl<3, 2>(e, coeffs, layer, writeIntegrated, stream);
-    case 4: return e.arithmetic == SDG_BF16X6 ? mfma::localImpl<4, 3>(e, coeffs, layer, writeIntegrated, stream)
-                                          : mfma::localImpl<4, 2>(e, coeffs, layer, writeIntegrated, stream);
-    case 5: return e.arithmetic == SDG_BF16X6 ? mfma::localImpl<5, 3>(e, coeffs, layer, writeIntegrated, stream)
-                                          : mfma::localImpl<5, 2>(e, coeffs, layer, writeIntegrated, stream);
-    case 6: return e.arithmetic == SDG_BF16X6 ? mfma::localImpl<6, 3>(e, coeffs, layer, writeIntegrated, stream)
-                                          : mfma::localImpl<6, 2>(e, coeffs, layer, writeIntegrated, stream);
-    case 7: return e.arithmetic == SDG_BF16X6 ? mfma::localImpl<7, 3>(e, coeffs, layer, writeIntegrated, stream)
-                                          : mfma::localImpl<7, 2>(e, coeffs, layer, writeIntegrated, stream);
+    case 3: return mfma::localDispatch<3>(e, coeffs, layer, writeIntegrated, stream);
+    case 4: return mfma::localDispatch<4>(e, coeffs, layer, writeIntegrated, stream);
+    case 5: return mfma::localDispatch<5>(e, coeffs, layer, writeIntegrated, stream);
+    case 6: return mfma::localDispatch<6>(e, coeffs, layer, writeIntegrated, stream);
+    case 7: return mfma::localDispatch<7>(e, coeffs, layer, writeIntegrated, stream);
     default: setError("matrix-core path supports orders 3..7"); return SDG_ERR_UNSUPPORTED;
   }
 }
 
 int launchNeighborMfma(const EngineState& e, const sdg_layer* layer, hipStream_t stream) {
   switch (e.order) {
-    case 3: return e.arithmetic == SDG_BF16X6 ? mfma::neighborImpl<3, 3>(e, layer, stream)
-                                          : mfma::neighborImpl<3, 2>(e, layer, stream);
-    case 4: return e.arithmetic == SDG_BF16X6 ? mfma::neighborImpl<4, 3>(e, layer, stream)
-                                          : mfma::neighborImpl<4, 2>(e, layer, stream);
-    case 5: return e.arithmetic == SDG_BF16X6 ? mfma::neighborImpl<5, 3>(e, layer, stream)
-                                          : mfma::neighborImpl<5, 2>(e, layer, stream);
-    case 6: return e.arithmetic == SDG_BF16X6 ? mfma::neighborImpl<6, 3>(e, layer, stream)
-                                          : mfma::neighborImpl<6, 2>(e, layer, stream);
-    case 7: return e.arithmetic == SDG_BF16X6 ? mfma::neighborImpl<7, 3>(e, layer, stream)
-                                          : mfma::neighborImpl<7, 2>(e, layer, stream);
+    case 3: return mfma::neighborDispatch<3>(e, layer, stream);
+    case 4: return mfma::neighborDispatch<4>(e, layer, stream);
+    case 5: return mfma::neighborDispatch<5>(e, layer, stream);
+    case 6: return mfma::neighborDispatch<6>(e, layer, stream);
+    case 7: return mfma::neighborDispatch<7>(e, layer, stream);
     default: setError("matrix-core path supports orders 3..7"); return SDG_ERR_UNSUPPORTED;
   }
 }

@@ -85,7 +85,7 @@ using namespace sdg;
 
 extern "C" int sdg_performance_estimate(int order, int arithmetic, int kernel_id, sdg_work_estimate* out) {
   if (out == nullptr || order < kMinOrder || order > kMaxOrder || arithmetic < SDG_FP64 ||
-      arithmetic > SDG_BF16X6 || kernel_id < 0 || kernel_id > 4) {
+      arithmetic > SDG_F16X3 || kernel_id < 0 || kernel_id > 4) {
     setError("sdg_performance_estimate: invalid order, arithmetic, kernel or NULL output");
     return SDG_ERR_INVALID;
   }
@@ -97,8 +97,8 @@ extern "C" int sdg_performance_estimate(int order, int arithmetic, int kernel_id
   const Work w = elementWork(order, gm);
   const int B = numBasis3(order), B2 = numBasis2(order);
   const double s = arithmetic == SDG_FP64 ? 8.0 : 4.0;
-  const double products = arithmetic == SDG_BF16X3 ? 3.0 : (arithmetic == SDG_BF16X6 ? 6.0 : 1.0);
-  const bool traces = arithmetic == SDG_BF16X3 || arithmetic == SDG_BF16X6;
+  const double products = arithmetic == SDG_BF16X6 ? 6.0 : (matrixCore(arithmetic) ? 3.0 : 1.0);
+  const bool traces = matrixCore(arithmetic);
   const double q = 9.0 * B, traceBytes = 4.0 * 9 * B2 * 4;
   Counts c;
   double bytes = 0, kernelBytes = 0;

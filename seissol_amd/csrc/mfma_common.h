@@ -10,8 +10,8 @@ constexpr int basis3(int o) { return o * (o + 1) * (o + 2) / 6; }
 constexpr int basis2(int o) { return o * (o + 1) / 2; }
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-// bf16 elements of one (m-tile, k-step) fragment set: 64 lanes x 8 values for
-// each of the three split planes (hi, mid, lo), stored one plane after the other.
+// 16-bit elements (bf16 or fp16) of one (m-tile, k-step) fragment set: 64
+// lanes x 8 values for each of three split planes, one plane after the other.
 constexpr int kFragPlane = 512;
 constexpr int kFragPair = 3 * kFragPlane;
 constexpr int kTileElems = 16;        // elements per workgroup tile
@@ -37,10 +37,13 @@ struct MShape {
   static constexpr int64_t fmrt = vol + 3LL * MT * KSB * kFragPair;  // [MTF][KSB]
   static constexpr int64_t rdm = fmrt + 1LL * MTF * KSB * kFragPair; // [MT][KSF]
   static constexpr int64_t total = rdm + 1LL * MT * KSF * kFragPair;
-  // float tables: invM2[B2], fP[3][B2][B2]
+  // float tables: invM2[B2], fP[3][B2][B2], unscale factors of the stationary
+  // matrices [ck0, ck1, ck2, vol0, vol1, vol2, fmrt, rdm] (1 unless the
+  // fragments were scaled by a power of two)
   static constexpr int tInvM2 = 0;
   static constexpr int tFP = B2;
-  static constexpr int tTotal = B2 + 3 * B2 * B2;
+  static constexpr int tAUn = B2 + 3 * B2 * B2;
+  static constexpr int tTotal = tAUn + 8;
 };
 
 }  // namespace mfma

@@ -2,6 +2,7 @@
 // engine lifetime, batched kernels and the proxy kernel selection.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <functional>
 #include <cstring>
@@ -83,26 +84,48 @@ float bf16Value(uint16_t h) {
   return f;
 }
 
-// Split a row-major FP64 matrix into three bf16 planes (x = hi + mid + lo,
-// each plane the round-to-nearest bf16 of the remainder) laid out in the lane
-// order of v_mfma_f32_16x16x32_bf16's A operand: lane l of fragment (mt, ks)
-// holds A[16 mt + l % 16][32 ks + 8 (l / 16) + i], i < 8.
+uint16_t f16Bits(float f) {
+  const _Float16 h = static_cast<_Float16>(f);  // round to nearest even
+  uint16_t b;
+  std::memcpy(&b, &h, 2);
+  return b;
+}
+float f16Value(uint16_t b) {
+  _Float16 h;
+  std::memcpy(&h, &b, 2);
+  return static_cast<float>(h);
+}
+
+// Split a row-major FP64 matrix (times `scale`) into three planes of bf16 or
+// fp16 (x = hi + mid + lo, each plane the round-to-nearest of the remainder)
+// laid out in the lane order of v_mfma_f32_16x16x32_{bf16,f16}'s A operand:
+// lane l of fragment (mt, ks) holds A[16 mt + l % 16][32 ks + 8 (l / 16) + i], i < 8.
 void appendFragments(const std::function<double(int, int)>& A, int rows, int cols, int mtiles, int ksteps,
-                     uint16_t* out) {
+                     uint16_t* out, bool f16, double scale) {
   for (int mt = 0; mt < mtiles; ++mt)
     for (int ks = 0; ks < ksteps; ++ks) {
       uint16_t* base = out + (static_cast<int64_t>(mt) * ksteps + ks) * mfma::kFragPair;
       for (int l = 0; l < 64; ++l)
         for (int i = 0; i < 8; ++i) {
           const int r = 16 * mt + l % 16, c = 32 * ks + 8 * (l / 16) + i;
-          double rest = (r < rows && c < cols) ? A(r, c) : 0.0;
+          double rest = (r < rows && c < cols) ? A(r, c) * scale : 0.0;
           for (int plane = 0; plane < 3; ++plane) {
-            const uint16_t h = bf16Bits(static_cast<float>(rest));
+            const uint16_t h = f16 ? f16Bits(static_cast<float>(rest)) : bf16Bits(static_cast<float>(rest));
             base[plane * mfma::kFragPlane + l * 8 + i] = h;
-            rest -= static_cast<double>(bf16Value(h));
+            rest -= static_cast<double>(f16 ? f16Value(h) : bf16Value(h));
           }
         }
     }
+}
+
+// power of two that brings the largest |entry| into [2^13, 2^14) (fp16 form), else 1
+double fragmentScale(const std::function<double(int, int)>& A, int rows, int cols, bool f16) {
+  if (!f16) return 1.0;
+  double mx = 0.0;
+  for (int r = 0; r < rows; ++r)
+    for (int c = 0; c < cols; ++c) mx = std::max(mx, std::fabs(A(r, c)));
+  if (mx == 0.0) return 1.0;
+  return std::ldexp(1.0, 13 - std::ilogb(mx));
 }
 
 template <int O>
@@ -110,18 +133,25 @@ int uploadMfmaData(EngineState& s, const double* gm) {
   using S = mfma::MShape<O>;
   const auto g = globalMatrixLayout(O);
   const int B = S::B, B2 = S::B2;
+  const bool f16 = s.arithmetic == SDG_F16X3;
   std::vector<uint16_t> frags(S::total, 0);
-  for (int j = 0; j < 3; ++j) {
-    appendFragments([&](int r, int c) { return -gm[g.kDivMT + (j * B + r) * B + c]; }, B, B, S::MT, S::KSB,
-                    frags.data() + S::ck + static_cast<int64_t>(j) * S::MT * S::KSB * mfma::kFragPair);
-    appendFragments([&](int r, int c) { return gm[g.kDivM + (j * B + r) * B + c]; }, B, B, S::MT, S::KSB,
-                    frags.data() + S::vol + static_cast<int64_t>(j) * S::MT * S::KSB * mfma::kFragPair);
-  }
-  appendFragments([&](int r, int c) { return gm[g.fMrT + ((r / B2) * B2 + r % B2) * B + c]; }, S::FR, B, S::MTF,
-                  S::KSB, frags.data() + S::fmrt);
-  appendFragments([&](int r, int c) { return gm[g.rDivM + ((c / B2) * B + r) * B2 + c % B2]; }, B, S::FR, S::MT,
-                  S::KSF, frags.data() + S::rdm);
   std::vector<float> tables(S::tTotal);
+  auto add = [&](const std::function<double(int, int)>& A, int rows, int cols, int mtiles, int ksteps,
+                 int64_t offset, int unscaleSlot) {
+    const double sc = fragmentScale(A, rows, cols, f16);
+    appendFragments(A, rows, cols, mtiles, ksteps, frags.data() + offset, f16, sc);
+    tables[S::tAUn + unscaleSlot] = static_cast<float>(1.0 / sc);
+  };
+  for (int j = 0; j < 3; ++j) {
+    add([&](int r, int c) { return -gm[g.kDivMT + (j * B + r) * B + c]; }, B, B, S::MT, S::KSB,
+        S::ck + static_cast<int64_t>(j) * S::MT * S::KSB * mfma::kFragPair, j);
+    add([&](int r, int c) { return gm[g.kDivM + (j * B + r) * B + c]; }, B, B, S::MT, S::KSB,
+        S::vol + static_cast<int64_t>(j) * S::MT * S::KSB * mfma::kFragPair, 3 + j);
+  }
+  add([&](int r, int c) { return gm[g.fMrT + ((r / B2) * B2 + r % B2) * B + c]; }, S::FR, B, S::MTF, S::KSB,
+      S::fmrt, 6);
+  add([&](int r, int c) { return gm[g.rDivM + ((c / B2) * B + r) * B2 + c % B2]; }, B, S::FR, S::MT, S::KSF,
+      S::rdm, 7);
   for (int m = 0; m < B2; ++m) tables[S::tInvM2 + m] = static_cast<float>(1.0 / gm[g.mass2 + m]);
   for (int i = 0; i < 3 * B2 * B2; ++i) tables[S::tFP + i] = static_cast<float>(gm[g.fP + i]);
   if (s.mfmaFrags == nullptr) SDG_HIP_CHECK(hipMalloc(&s.mfmaFrags, frags.size() * sizeof(uint16_t)));
@@ -313,7 +343,7 @@ int sdg_engine_create(sdg_engine** engine, int order, int arithmetic) {
     setError("sdg_engine_create: order must be in [2, 7]");
     return SDG_ERR_UNSUPPORTED;
   }
-  if (arithmetic != SDG_FP64 && arithmetic != SDG_FP32 && arithmetic != SDG_BF16X3 && arithmetic != SDG_BF16X6) {
+  if (arithmetic < SDG_FP64 || arithmetic > SDG_F16X3) {
     setError("sdg_engine_create: unknown arithmetic");
     return SDG_ERR_INVALID;
   }
@@ -323,7 +353,7 @@ int sdg_engine_create(sdg_engine** engine, int order, int arithmetic) {
   std::vector<double> gm(globalMatrixLayout(order).total);
   generateGlobalMatrices(order, gm.data());
   int rc = uploadMatrices(e->s, gm.data());
-  if (rc == SDG_OK && (arithmetic == SDG_BF16X3 || arithmetic == SDG_BF16X6)) rc = uploadMfma(e->s, gm.data());
+  if (rc == SDG_OK && matrixCore(arithmetic)) rc = uploadMfma(e->s, gm.data());
   if (rc != SDG_OK) {
     if (e->s.matrices) (void)hipFree(e->s.matrices);
     if (e->s.mfmaFrags) (void)hipFree(e->s.mfmaFrags);
@@ -338,7 +368,7 @@ int sdg_engine_create(sdg_engine** engine, int order, int arithmetic) {
 int sdg_engine_set_global_matrices(sdg_engine* engine, const double* host_matrices) {
   if (engine == nullptr || host_matrices == nullptr) { setError("invalid arguments"); return SDG_ERR_INVALID; }
   const int rc = uploadMatrices(engine->s, host_matrices);
-  if (rc != SDG_OK || (engine->s.arithmetic != SDG_BF16X3 && engine->s.arithmetic != SDG_BF16X6)) return rc;
+  if (rc != SDG_OK || !matrixCore(engine->s.arithmetic)) return rc;
   return uploadMfma(engine->s, host_matrices);
 }
 
@@ -365,7 +395,7 @@ int sdg_compute_batched_ader_local(sdg_engine* engine, const double* coeffs, dou
     setError("star/aplus matrices missing");
     return SDG_ERR_INVALID;
   }
-  if ((engine->s.arithmetic == SDG_BF16X3 || engine->s.arithmetic == SDG_BF16X6) && layer->face_traces != nullptr)
+  if (matrixCore(engine->s.arithmetic) && layer->face_traces != nullptr)
     return launchLocalMfma(engine->s, coeffs, layer, true, static_cast<hipStream_t>(stream));
   return launchAderLocal(engine->s, coeffs, layer, static_cast<hipStream_t>(stream));
 }
@@ -392,7 +422,7 @@ int sdg_compute_batched_integral(sdg_engine* engine, const sdg_layer* layer, voi
 
 int sdg_compute_batched_neighbors_integral(sdg_engine* engine, const sdg_layer* layer, void* stream) {
   if (engine == nullptr || !validLayer(layer, true)) return SDG_ERR_INVALID;
-  if ((engine->s.arithmetic == SDG_BF16X3 || engine->s.arithmetic == SDG_BF16X6) && layer->face_traces != nullptr)
+  if (matrixCore(engine->s.arithmetic) && layer->face_traces != nullptr)
     return launchNeighborMfma(engine->s, layer, static_cast<hipStream_t>(stream));
   return launchNeighbor(engine->s, layer, static_cast<hipStream_t>(stream));
 }
@@ -413,7 +443,7 @@ int sdg_proxy_run(sdg_engine* engine, int kernel_id, int steps, double dt, const
   std::vector<double> coeffs(engine->s.order);
   sdg_monomial_integrate(engine->s.order, 0.0, dt, coeffs.data());
   auto st = static_cast<hipStream_t>(stream);
-  const bool mc = engine->s.arithmetic == SDG_BF16X3 || engine->s.arithmetic == SDG_BF16X6;
+  const bool mc = matrixCore(engine->s.arithmetic);
   if (mc && layer->num_elements > 0 && layer->face_traces == nullptr && (kernel_id == 0 || kernel_id == 1 || kernel_id == 2)) {
     setError("sdg_proxy_run: matrix-core arithmetic needs layer->face_traces");
     return SDG_ERR_INVALID;

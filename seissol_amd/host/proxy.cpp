@@ -1,7 +1,7 @@
 // seissol_amd_proxy: the SeisSol proxy benchmark on seissol_amd.
 //
 // Usage (as app/Proxy/Main.cpp:49-66 plus this engine's options):
-//   seissol_amd_proxy [--order O] [--arithmetic fp64|fp32|bf16x3|bf16x6]
+//   seissol_amd_proxy [--order O] [--arithmetic fp64|fp32|bf16x3|bf16x6|f16x3]
 //                     [--mesh proxy|periodic] [--seed S] [-f plain|json]
 //                     <cells> <timesteps> <kernel[,kernel...]>
 // Kernels: all, local, neigh, ader, localwoader (src/Proxy/Common.cpp:22-41);
@@ -84,7 +84,7 @@ std::vector<std::string> split(const std::string& s) {
 
 int usage(const char* msg) {
   if (msg != nullptr) std::cerr << msg << "\n";
-  std::cerr << "usage: seissol_amd_proxy [--order O] [--arithmetic fp64|fp32|bf16x3|bf16x6] "
+  std::cerr << "usage: seissol_amd_proxy [--order O] [--arithmetic fp64|fp32|bf16x3|bf16x6|f16x3] "
                "[--mesh proxy|periodic] [--seed S] [-f plain|json] <cells> <timesteps> <kernel[,kernel...]>\n"
                "kernels: all, local, neigh, ader, localwoader\n";
   return -1;
@@ -217,7 +217,8 @@ int run(int argc, char** argv) {
   check(sdg_upload_reals(global.engine(), dDofs, dofs.data(), static_cast<int64_t>(dofs.size()), nullptr),
         "sdg_upload_reals");
   void* dInt = deviceZeros(n * 9 * B * real);
-  const bool traces = arithmetic == Arithmetic::BF16x3 || arithmetic == Arithmetic::BF16x6;
+  const bool traces = arithmetic == Arithmetic::BF16x3 || arithmetic == Arithmetic::BF16x6 ||
+                      arithmetic == Arithmetic::F16x3;
   void* dTraces = traces ? deviceZeros(n * 4 * 9 * B2 * 4) : nullptr;
   hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
   sdg_layer layer{cells, dDofs, dInt, nullptr, star, aplus, aminus, dNb, dRel, dTypes, dTraces};

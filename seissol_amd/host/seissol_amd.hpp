@@ -29,14 +29,21 @@ inline void check(int rc, const char* what) {
   if (rc != SDG_OK) throw std::runtime_error(std::string(what) + ": " + sdg_last_error());
 }
 
-enum class Arithmetic : int { FP64 = SDG_FP64, FP32 = SDG_FP32, BF16x3 = SDG_BF16X3, BF16x6 = SDG_BF16X6 };
+enum class Arithmetic : int {
+  FP64 = SDG_FP64,
+  FP32 = SDG_FP32,
+  BF16x3 = SDG_BF16X3,
+  BF16x6 = SDG_BF16X6,
+  F16x3 = SDG_F16X3
+};
 
 inline Arithmetic arithmeticFromString(const std::string& s) {
   if (s == "fp64") return Arithmetic::FP64;
   if (s == "fp32") return Arithmetic::FP32;
   if (s == "bf16x3") return Arithmetic::BF16x3;
   if (s == "bf16x6") return Arithmetic::BF16x6;
-  throw std::runtime_error("unknown arithmetic " + s + " (fp64, fp32, bf16x3, bf16x6)");
+  if (s == "f16x3") return Arithmetic::F16x3;
+  throw std::runtime_error("unknown arithmetic " + s + " (fp64, fp32, bf16x3, bf16x6, f16x3)");
 }
 
 /// Stream on which the batched kernels are enqueued (the reference's StreamRuntime).

@@ -21,7 +21,8 @@ from . import _capi
 from ._capi import Layer, check, lib, ptr
 
 ARITHMETIC = {"fp64": _capi.SDG_FP64, "fp32": _capi.SDG_FP32, "bf16x3": _capi.SDG_BF16X3,
-              "bf16x6": _capi.SDG_BF16X6}
+              "bf16x6": _capi.SDG_BF16X6, "f16x3": _capi.SDG_F16X3}
+MATRIX_CORE = ("bf16x3", "bf16x6", "f16x3")
 
 
 def num_basis(order: int) -> int:

@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _capi
 from ._capi import Layer, check, lib, ptr
-from .kernels import Engine, derivatives_size, num_basis, num_face_basis
+from .kernels import MATRIX_CORE, Engine, derivatives_size, num_basis, num_face_basis
 
 KERNELS = ("all", "local", "neigh", "ader", "localwoader")
 
@@ -134,7 +134,7 @@ class DeviceLayer:
                             if with_derivatives else None)
         B2 = num_face_basis(setup.order)
         self.face_traces = (torch.zeros((n, 4, 9, B2), dtype=torch.float32, device=dev)
-                            if engine.arithmetic in ("bf16x3", "bf16x6") else None)
+                            if engine.arithmetic in MATRIX_CORE else None)
         self.view = Layer(n, ptr(self.dofs), ptr(self.time_integrated), ptr(self.derivatives), ptr(self.star),
                           ptr(self.aplus), ptr(self.aminus), ptr(self.neighbors), ptr(self.face_relations),
                           ptr(self.face_types), ptr(self.face_traces))

@@ -126,10 +126,14 @@ def test_warm_start_keeps_the_root_and_reports_iterations(gpu):
     train, held = buf[:n][perm[: n * 9 // 10]].contiguous(), buf[:n][perm[n * 9 // 10:]].contiguous()
     tr = WarmStartTrainer(seed=0)
     tr.fit(train, epochs=8, batch=2048, lr=3e-3)
-    # the network improves on the closed-form creep estimate it corrects
-    err = (tr.predict(held) - held[:, 6]).abs().median().item()
-    err_base = held[:, 6].abs().median().item()
-    print(f"held-out median |log10 error|: network {err:.4f}, creep estimate {err_base:.4f}", flush=True)
+    # the network improves on the closed-form creep estimate it corrects, where
+    # that estimate is off (elsewhere it is exact to rounding)
+    off = held[:, 6].abs() > 0.01
+    assert int(off.sum().item()) > 100
+    err = (tr.predict(held) - held[:, 6]).abs()[off].median().item()
+    err_base = held[:, 6].abs()[off].median().item()
+    print(f"held-out median |log10 error| where the creep estimate is off by > 0.01: network {err:.4f}, "
+          f"creep estimate {err_base:.4f}", flush=True)
     assert err < 0.5 * err_base
     # warm-started solver from the same initial state
     solver2 = FrictionSolver(setup.params, 4)

@@ -197,9 +197,9 @@ def test_empty_layer_and_errors(gpu):
 
 
 # ---------------------------------------------------------------------------
-# matrix-core arithmetic (FP32 data, split-BF16 products, FP32 accumulation)
+# matrix-core arithmetic (FP32 data, split bf16/fp16 products, FP32 accumulation)
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("arith", ["bf16x3", "bf16x6"])
+@pytest.mark.parametrize("arith", ["bf16x3", "bf16x6", "f16x3"])
 @pytest.mark.parametrize("order,mesh", [(3, "proxy"), (4, "proxy"), (5, "periodic"), (6, "proxy"), (6, "periodic"),
                                         (7, "periodic")])
 def test_split_bf16_proxy_all_matches_oracle(gpu, order, mesh, arith):
@@ -218,11 +218,12 @@ def test_split_bf16_proxy_all_matches_oracle(gpu, order, mesh, arith):
                                 setup.face_types)
     err = rel_l2(_np(layer.dofs), q_ref)
     print(f"{arith} order {order} {mesh}: rel L2 after {steps} steps {err:.2e}")
-    # 2-plane splits: ~2^-17 per contraction; 3-plane splits: FP32 level
-    assert err < (2e-5 if arith == "bf16x3" else 2e-6), err
+    # 2-plane bf16 splits: ~2^-17 per contraction; 2-plane scaled fp16: ~2^-21;
+    # 3-plane bf16 splits: FP32 level
+    assert err < {"bf16x3": 2e-5, "f16x3": 5e-6, "bf16x6": 2e-6}[arith], err
 
 
-@pytest.mark.parametrize("arith", ["bf16x3", "bf16x6"])
+@pytest.mark.parametrize("arith", ["bf16x3", "bf16x6", "f16x3"])
 def test_split_bf16_local_writes_integral_and_traces(gpu, arith):
     import torch
     order = 6
@@ -234,8 +235,8 @@ def test_split_bf16_local_writes_integral_and_traces(gpu, arith):
     _sync()
     gm = global_matrices(order)
     I_ref = oracle.ader(order, gm["flat"], coeffs, setup.dofs, _np(layer.star64))
-    # split-BF16 products: about 2^-17 relative per contraction
-    tol = 1e-5 if arith == "bf16x3" else 1e-6
+    # split products: about 2^-17 (bf16x3), 2^-21 (f16x3), 2^-24 (bf16x6) per contraction
+    tol = {"bf16x3": 1e-5, "f16x3": 3e-6, "bf16x6": 1e-6}[arith]
     err_i = rel_l2(_np(layer.time_integrated), I_ref)
     assert err_i < tol, err_i
     q_ref = oracle.local(order, gm["flat"], I_ref, _np(layer.star64), _np(layer.aplus64), setup.dofs)
