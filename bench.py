@@ -1,15 +1,19 @@
 """Benchmark of the ADER-DG element update (proxy kernel `all`) on one GPU.
 
 Workload (default): elastic ORDER 6 (56 basis functions x 9 quantities),
-400,000 tetrahedra of a conforming periodic jittered box (40 x 40 x 50 cubes,
+1,000,000 tetrahedra of a conforming periodic jittered box (50 x 50 x 80 cubes,
 5 tets each), rho ~ U[2000, 3000], vp 6000, vs 3464, DOFs ~ U[-1, 1], dt from
-CFL 0.5, global time stepping; one step = ader + local integral + neighbour
-integral of every element.  Inputs are resident in HBM before timing starts.
+CFL 0.5, global time stepping, 200 steps, in the chosen split-fp16 matrix-core
+form; one step = ader + local integral + neighbour integral of every element.
+Inputs are resident in HBM before timing starts.
 
-Prints one JSON line (rank 0).  Besides the throughput it reports the roofline
-of the dominant kernel (algorithmic bytes / measured average launch time, HIP
-events on the launch stream) and the CPU FP64 oracle timed on a bounded sample
-of the same workload on the host cores.
+Prints one JSON line (rank 0).  Besides the throughput it reports the plain
+FP64 path on the same input (throughput and the per-quantity relative L2
+error of the form against it after the same steps), the roofline of the
+dominant kernel (algorithmic bytes / measured average launch time, HIP events
+on the launch stream; HBM traffic per launch from the rocprofv3 counter
+summaries under profiles/ when one matches the workload) and the CPU FP64
+oracle timed on a bounded sample of the same workload on the host cores.
 """
 from __future__ import annotations
 
@@ -59,6 +63,21 @@ def cpu_baseline(order: int, elements: int, steps: int, seed: int) -> dict:
                       f"(CPU restatement, not the reference binary), {sec:.1f} s"}
 
 
+def measured_traffic(form: str, order: int, elements: int, kernel: str):
+    """HBM bytes per launch of `kernel` from a counter summary under profiles/
+    (a line `bench-traffic: form=F order=O elements=N kernel=K bytes=B`), or None."""
+    from pathlib import Path
+    want = {"form": form, "order": str(order), "elements": str(elements), "kernel": kernel}
+    for path in sorted((Path(__file__).resolve().parent / "profiles").glob("*.md")):
+        for line in path.read_text().splitlines():
+            if not line.startswith("bench-traffic:"):
+                continue
+            fields = dict(kv.split("=", 1) for kv in line.split(":", 1)[1].split())
+            if all(fields.get(k) == v for k, v in want.items()):
+                return float(fields["bytes"])
+    return None
+
+
 def friction_bench(args) -> dict:
     """Rate-and-state friction (law 103) at ORDER 4: plain Newton against the
     MLP warm start, counted in residual evaluations, on the same fault."""
@@ -85,8 +104,10 @@ def friction_bench(args) -> dict:
         torch.cuda.synchronize()
         return drv, s.elapsed_time(e) / steps
 
-    drv0, ms_plain = run(FrictionSolver(setup.params, 4), FrictionOptions(None, ptr(buf), cap, ptr(cnt), stride, 0))
+    # plain solver: one run recording training samples, one timed run without
+    run(FrictionSolver(setup.params, 4), FrictionOptions(None, ptr(buf), cap, ptr(cnt), stride, 0))
     n = min(int(cnt.item()), cap)
+    drv0, ms_plain = run(FrictionSolver(setup.params, 4))
     plain = int(drv0.fault.t["newton_iterations"].sum().item())
     log(f"friction plain: {plain} evaluations, {ms_plain:.3f} ms/step, {n} samples")
     g = torch.Generator(device="cuda").manual_seed(args.seed)
@@ -94,7 +115,7 @@ def friction_bench(args) -> dict:
     train, held = buf[:n][perm[: n * 19 // 20]].contiguous(), buf[:n][perm[n * 19 // 20:]].contiguous()
     tr = WarmStartTrainer(seed=args.seed)
     t0 = time.perf_counter()
-    tr.fit(train, epochs=args.friction_epochs, batch=4096, lr=3e-3)
+    tr.fit(train, steps=args.friction_train_steps, batch=1024, lr=3e-3, seed=args.seed)
     train_s = time.perf_counter() - t0
     held_err = (tr.predict(held) - held[:, 6]).abs()
     off = held[:, 6].abs() > 0.01
@@ -115,7 +136,7 @@ def friction_bench(args) -> dict:
         "config": {"workload": f"planar fault, {args.friction_faces} faces x {setup.num_points} Gauss points, "
                                f"ORDER 4, {steps} DR steps of {dt} s, nucleation + travelling shear pulse",
                    "faces": args.friction_faces, "points": setup.num_points, "steps": steps, "seed": args.seed},
-        "training": {"samples": n, "stride": stride, "epochs": args.friction_epochs, "seconds": train_s,
+        "training": {"samples": n, "stride": stride, "adam_steps": args.friction_train_steps, "seconds": train_s,
                      "held_out_median_abs_log10_err": float(held_err.median().item()),
                      "held_out_p90_abs_log10_err": float(held_err.quantile(0.9).item()),
                      "creep_estimate_median_abs_log10_err": float(held[:, 6].abs().median().item()),
@@ -130,18 +151,64 @@ def friction_bench(args) -> dict:
     }
 
 
+def lts_bench(args) -> dict:
+    """Rate-2 local time stepping (clusters 60/25/10/5 %, dt ratios 1:2:4:8),
+    macro steps enqueued eagerly and replayed from a captured hipGraph."""
+    import torch
+
+    from seissol_amd.kernels import Engine
+    from seissol_amd.lts import DeviceLtsLayer, LocalTimeStepping, make_lts_setup
+
+    t0 = time.perf_counter()
+    lts = make_lts_setup(args.lts_order, n=args.lts_elements, seed=args.seed)
+    log(f"lts setup {lts.setup.n} elements {lts.setup.dims}, clusters {np.diff(lts.cluster_start).tolist()}, "
+        f"dt0 {lts.dt0:.3e} in {time.perf_counter() - t0:.1f} s")
+    eng = Engine(args.lts_order, args.lts_form)
+    layer = DeviceLtsLayer(eng, lts)
+    run = LocalTimeStepping(eng, layer)
+    upd = lts.updates_per_macro_step()
+    out = {}
+    for mode in ("eager", "graph"):
+        graph = mode == "graph"
+        run.advance(1, graph=graph)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        run.advance(args.lts_steps, graph=graph)
+        e.record()
+        torch.cuda.synchronize()
+        ms = s.elapsed_time(e) / args.lts_steps
+        out[mode] = {"ms_per_macro_step": ms, "element_updates_per_s": upd / (ms / 1e3)}
+        log(f"lts {mode}: {ms:.3f} ms per macro step -> {upd / (ms / 1e3) / 1e6:.2f} M element-updates/s")
+    finite = bool(torch.isfinite(layer.dofs).all().item())
+    C_ = lts.num_clusters
+    gts_updates = lts.setup.n * (1 << (C_ - 1))  # the same simulated time at dt0 everywhere
+    return {
+        "metric": "element-updates/s, rate-2 LTS clusters, ORDER 5 elastic",
+        "value": out["graph"]["element_updates_per_s"], "unit": "element-updates/s", "higher_is_better": True,
+        "modes": out, "dtype": {"fp64": "f64", "fp32": "f32"}[args.lts_form],
+        "config": {"workload": f"ORDER {args.lts_order} elastic, {lts.setup.n} tetrahedra (periodic box "
+                               f"{lts.setup.dims}), clusters {np.diff(lts.cluster_start).tolist()} with dt0 * 2^c, "
+                               f"{args.lts_steps} macro steps",
+                   "cluster_sizes": np.diff(lts.cluster_start).tolist(), "dt0": lts.dt0,
+                   "updates_per_macro_step": upd, "launches_per_macro_step": 2 * ((1 << C_) - 1),
+                   "updates_saved_vs_global_dt0": gts_updates / upd, "finite": finite},
+        "data": "synthetic",
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--order", type=int, default=6)
-    ap.add_argument("--elements", type=int, default=400_000)
-    ap.add_argument("--form", choices=["fp64", "fp32", "bf16x3", "bf16x6", "f16x3"], default="bf16x6")
+    ap.add_argument("--elements", type=int, default=1_000_000)
+    ap.add_argument("--form", choices=["fp64", "fp32", "bf16x3", "bf16x6", "f16x3"], default="f16x3")
     ap.add_argument("--mesh", choices=["periodic", "proxy"], default="periodic")
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--cpu-elements", type=int, default=20_000)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-elements", type=int, default=100_000)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error-check", action="store_true",
                     help="skip the rel. L2 comparison of the form against the plain FP64 path")
@@ -150,7 +217,12 @@ def main(argv=None) -> int:
     ap.add_argument("--friction-faces", type=int, default=20_000)
     ap.add_argument("--friction-steps", type=int, default=60)
     ap.add_argument("--friction-samples", type=int, default=1_000_000)
-    ap.add_argument("--friction-epochs", type=int, default=6)
+    ap.add_argument("--friction-train-steps", type=int, default=4000)
+    ap.add_argument("--lts", action="store_true", help="only run the local-time-stepping benchmark")
+    ap.add_argument("--lts-order", type=int, default=5)
+    ap.add_argument("--lts-elements", type=int, default=600_000)
+    ap.add_argument("--lts-steps", type=int, default=10)
+    ap.add_argument("--lts-form", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -174,12 +246,16 @@ def main(argv=None) -> int:
     if args.friction:
         print(json.dumps(friction_bench(args)), flush=True)
         return 0
+    if args.lts:
+        print(json.dumps(lts_bench(args)), flush=True)
+        return 0
 
     from seissol_amd.kernels import Engine
     from seissol_amd.proxy import DeviceLayer, make_setup, run_proxy
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
-    dims = (40, 40, 50) if (args.mesh == "periodic" and args.elements == 400_000) else None
+    known = {400_000: (40, 40, 50), 1_000_000: (50, 50, 80)}
+    dims = known.get(args.elements) if args.mesh == "periodic" else None
     t0 = time.perf_counter()
     setup = make_setup(args.order, n=args.elements, mesh=args.mesh, seed=args.seed, dims=dims)
     log(f"setup {setup.n} elements ({args.mesh}) in {time.perf_counter() - t0:.1f} s, dt={setup.dt:.3e}")
@@ -217,8 +293,9 @@ def main(argv=None) -> int:
     log(f"per launch: local {per_kernel['local']:.3f} ms, neigh {per_kernel['neigh']:.3f} ms")
 
     # accuracy of the form: same initial state, same steps, against the plain
-    # FP64 path (itself pinned to the CPU oracle by the parity tests)
-    error = None
+    # FP64 path (itself pinned to the CPU oracle by the parity tests), and the
+    # plain FP64 path's own throughput on that run
+    error, plain = None, None
     if args.form != "fp64" and not args.no_error_check:
         layer.reset_dofs(setup.dofs)
         run_proxy(eng, layer, "all", args.steps, setup.dt)
@@ -226,15 +303,33 @@ def main(argv=None) -> int:
         del layer
         eng64 = Engine(args.order, "fp64")
         layer64 = DeviceLayer(eng64, setup)
-        run_proxy(eng64, layer64, "all", args.steps, setup.dt)
         torch.cuda.synchronize()
+        s64, e64 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s64.record()
+        run_proxy(eng64, layer64, "all", args.steps, setup.dt)
+        e64.record()
+        torch.cuda.synchronize()
+        ms64 = s64.elapsed_time(e64)
+        plain = {"value": setup.n * args.steps / (ms64 / 1e3), "ms_per_step": ms64 / args.steps,
+                 "form": "fp64 (VALU, the drop-in)"}
         diff = (q_form - layer64.dofs).pow(2).sum(dim=(0, 2)).sqrt()
         ref = layer64.dofs.pow(2).sum(dim=(0, 2)).sqrt()
         per_q = (diff / ref).tolist()
         error = {"steps": args.steps, "rel_l2_per_quantity": per_q, "max": max(per_q),
                  "reference": "plain FP64 GPU path, same input"}
-        log(f"rel L2 vs FP64 after {args.steps} steps: max {max(per_q):.2e}")
+        log(f"plain FP64: {plain['value'] / 1e6:.2f} M element-updates/s; rel L2 vs FP64 after {args.steps} steps: "
+            f"max {max(per_q):.2e}")
         del layer64, q_form
+
+    # matrix-core work of the dominant kernel (dense products the form executes)
+    from seissol_amd._capi import WorkEstimate, check, lib
+    from seissol_amd.kernels import ARITHMETIC
+    import ctypes as C
+    est = WorkEstimate()
+    check(lib().sdg_performance_estimate(args.order, ARITHMETIC[args.form], 1 if dom == "local" else 2,
+                                         C.byref(est)), "sdg_performance_estimate")
+    mma_tflops = est.hardware_flop * setup.n / (per_kernel[dom] / 1e3) / 1e12
+    traffic = measured_traffic(args.form, args.order, setup.n, dom)
 
     result = {
         "metric": METRIC,
@@ -247,6 +342,8 @@ def main(argv=None) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
+        "plain_fp64": plain,
+        "speedup_vs_plain_fp64": (value / plain["value"]) if plain else None,
         "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "f32 (2-plane split-bf16 MFMA, f32 accumulate)",
                   "bf16x6": "f32 (3-plane split-bf16 MFMA, f32 accumulate)",
                   "f16x3": "f32 (2-plane split-fp16 MFMA on power-of-two scaled operands, f32 accumulate)"}[args.form],
@@ -257,8 +354,11 @@ def main(argv=None) -> int:
                    "seed": args.seed, "dt": setup.dt, "finite": finite, "error_vs_fp64": error},
         "roofline": {"bound": "hbm", "kernel": f"proxy `{dom}` ({'ader+volume+local flux' if dom == 'local' else 'neighbour flux'})",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_element": ab[dom],
-                     "launch_ms": per_kernel[dom], "other_launch_ms": {k: v for k, v in per_kernel.items() if k != dom}},
+                     "traffic": traffic, "bytes_per_element": ab[dom],
+                     "launch_ms": per_kernel[dom], "other_launch_ms": {k: v for k, v in per_kernel.items() if k != dom},
+                     "matrix_core": {"achieved_tflops": mma_tflops,
+                                     "peak_tflops": 2500.0 if args.form in ("bf16x3", "bf16x6", "f16x3") else None,
+                                     "hardware_flop_per_element": est.hardware_flop}},
     }
     if not args.no_cpu:
         log("cpu baseline ...")

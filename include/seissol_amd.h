@@ -116,7 +116,16 @@ typedef struct {
                                   /* required by the matrix-core arithmetics  */
                                   /* SDG_BF16X3/X6, SDG_F16X3 (written by the */
                                   /* local pass, read by the neighbour pass)  */
+  const uint16_t* lts_setup;      /* [n] LTS bits (SDG_LTS_*) or NULL: global */
+                                  /* time stepping                            */
 } sdg_layer;
+
+/* Local time stepping bits of a cell (src/Initializer/LtsSetup.h:18-110). */
+#define SDG_LTS_NEIGHBOR_HAS_DERIVATIVES(face) (1u << (face))      /* neighbour gives derivatives */
+#define SDG_LTS_NEIGHBOR_GTS_RELATION(face) (1u << (4 + (face)))   /* same cluster: time coeffs   */
+#define SDG_LTS_HAS_BUFFERS (1u << 8)         /* cell stores its time-integrated buffer          */
+#define SDG_LTS_HAS_DERIVATIVES (1u << 9)     /* cell stores its derivatives                     */
+#define SDG_LTS_ACCUMULATE_BUFFERS (1u << 10) /* buffer sums the steps of a larger neighbour step */
 
 /* Spacetime::computeBatchedAder (src/Kernels/LinearCK/Time.cpp:142-217):
  * CK derivatives of dofs, time integral into time_integrated, optional
@@ -137,6 +146,43 @@ int sdg_compute_batched_neighbors_integral(sdg_engine* engine, const sdg_layer* 
  * src/Proxy/KernelDevice.cpp:69-88): one pass over dofs. */
 int sdg_compute_batched_ader_local(sdg_engine* engine, const double* coeffs, double time_step_width,
                                    const sdg_layer* layer, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Local time stepping (rate 2).  FP64 and FP32 engines.                  */
+/* ---------------------------------------------------------------------- */
+/* deriveLtsSetups (src/Initializer/InitProcedure/Internal/LtsSetup.cpp:97-330,
+ * getLtsSetup + normalizeLtsSetup), host: LTS bits of every cell from the
+ * cluster ids (larger id = larger time step) of the cell and its neighbours;
+ * face neighbours must be at most one cluster apart. */
+int sdg_derive_lts_setups(int64_t n, const int32_t* cluster_ids, const int32_t* neighbors,
+                          const uint8_t* face_types, uint16_t* setups);
+/* TimeCluster::computeLocalIntegration (src/Solver/TimeStepping/TimeCluster.cpp:358-434):
+ * ader + local integral of the layer's cells; `time_integrated` receives the
+ * buffer of cells with SDG_LTS_HAS_BUFFERS (added to it for cells with
+ * SDG_LTS_ACCUMULATE_BUFFERS unless reset_buffers), `derivatives` those of
+ * cells with SDG_LTS_HAS_DERIVATIVES.  Requires lts_setup and derivatives. */
+int sdg_compute_local_integration(sdg_engine* engine, const double* coeffs, double time_step_width,
+                                  const sdg_layer* layer, int reset_buffers, void* stream);
+/* TimeCluster::computeNeighboringIntegration (TimeCluster.cpp:540-628) with
+ * TimeCommon::computeIntegrals (src/Kernels/TimeCommon.cpp:40-87): for each
+ * regular face the neighbour's buffer, or its derivatives integrated with
+ * time_coeffs (same cluster) or subtime_coeffs (larger neighbour step).
+ * `time_integrated`, `derivatives` are the arrays of all cells (indexed by
+ * face_neighbors); the other arrays are the layer's own. */
+int sdg_compute_neighboring_integration(sdg_engine* engine, const double* time_coeffs,
+                                        const double* subtime_coeffs, const sdg_layer* layer, void* stream);
+/* Rate-2 clusters of one layer (the TimeManager's clusters, src/Solver/TimeStepping):
+ * cluster c = cells [cluster_start[c], cluster_start[c+1]) advancing with
+ * dt0 * 2^c; the layer holds all clusters (neighbour indices global). */
+typedef struct sdg_lts sdg_lts;
+int sdg_lts_create(sdg_lts** lts, sdg_engine* engine, const sdg_layer* layer, int num_clusters,
+                   const int64_t* cluster_start, double dt0);
+/* Advance every cluster by `macro_steps` steps of the slowest cluster. */
+int sdg_lts_advance(sdg_lts* lts, int macro_steps, void* stream);
+/* Same, replaying a hipGraph of one macro step captured on first use
+ * (StreamRuntime::runGraph, src/Parallel/Runtime/Stream.h:163-190). */
+int sdg_lts_graph_advance(sdg_lts* lts, int macro_steps, void* stream);
+int sdg_lts_destroy(sdg_lts* lts);
 
 /* Proxy kernel selection (src/Proxy/Common.cpp:22-41, KernelDevice.cpp:144-165):
  * "all", "local", "neigh", "ader", "localwoader".  Runs `steps` repetitions

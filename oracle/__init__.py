@@ -39,6 +39,7 @@ def lib() -> C.CDLL:
             "oracle_local": [I, P, I64, P, P, P, P, P],
             "oracle_neighbor": [I, P, I64, P, P, P, P, P, P],
             "oracle_proxy_all": [I, P, I64, I, P, P, P, P, P, P, P, P, P],
+            "oracle_lts": [I, P, I64, I, I, P, C.c_double, P, P, P, P, P, P, P, P],
         }
         for name, args in sig.items():
             fn = getattr(h, name)
@@ -106,6 +107,19 @@ def neighbor(order, gm_flat, I, neighbors, face_relations, aminus, dofs, face_ty
     ft = None if face_types is None else _c(face_types, np.uint8)
     lib().oracle_neighbor(order, _p(gm_flat), q.shape[0], _p(_c(I)), _p(_c(neighbors, np.int32)),
                           _p(_c(face_relations, np.uint8)), _p(_c(aminus)), _p(ft), _p(q))
+    return q
+
+
+def lts(order, gm_flat, macro_steps, num_clusters, cluster_ids, dt0, setups, star, aplus, aminus, neighbors,
+        face_relations, dofs, face_types=None):
+    """Rate-2 local time stepping of `macro_steps` steps of the slowest cluster."""
+    q = _c(dofs).copy()
+    ft = None if face_types is None else _c(face_types, np.uint8)
+    rc = lib().oracle_lts(order, _p(gm_flat), q.shape[0], macro_steps, num_clusters, _p(_c(cluster_ids, np.int32)),
+                          float(dt0), _p(_c(setups, np.uint16)), _p(_c(star)), _p(_c(aplus)), _p(_c(aminus)),
+                          _p(_c(neighbors, np.int32)), _p(_c(face_relations, np.uint8)), _p(ft), _p(q))
+    if rc != 0:
+        raise RuntimeError("oracle_lts failed")
     return q
 
 

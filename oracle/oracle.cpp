@@ -479,4 +479,83 @@ int oracle_proxy_all(int order, const double* gm, std::int64_t n, int steps, con
   return 0;
 }
 
+// Local time stepping with rate-2 clusters (cluster c advances with
+// dt0 * 2^c), restated from TimeCluster::computeLocalIntegration /
+// computeNeighboringIntegration (src/Solver/TimeStepping/TimeCluster.cpp:358-628),
+// TimeCommon::computeIntegrals (src/Kernels/TimeCommon.cpp:40-87) and the
+// LtsSetup bits (src/Initializer/LtsSetup.h:18-110): bits 0-3 neighbour
+// provides derivatives, 4-7 same-cluster relation, 8 buffers, 9 derivatives,
+// 10 accumulated buffers.  Time-driven order: at every multiple k of dt0 the
+// cells whose step begins at k predict (ader + local integral; buffer stored,
+// or added for accumulating cells on the second of two steps), and the cells
+// whose step ends at k + 1 correct (neighbour integral from the neighbour's
+// buffer, or its derivatives over [k_s - T, k_s - T + dt_c], T the start of
+// the neighbour's step).  Cells need not be sorted by cluster.
+int oracle_lts(int order, const double* gm, std::int64_t n, int macroSteps, int numClusters,
+               const std::int32_t* clusterIds, double dt0, const std::uint16_t* setups, const double* star,
+               const double* aplus, const double* aminus, const std::int32_t* neighbors,
+               const std::uint8_t* faceRelations, const std::uint8_t* faceTypes, double* dofs) {
+  const int B = nb3(order);
+  const std::int64_t q = 9LL * B, dsize = oracle_derivatives_size(order);
+  std::vector<double> buffers(n * q, 0.0), derivs(n * dsize, 0.0);
+  const std::int64_t fine = std::int64_t(1) << (numClusters - 1);
+  auto integrate = [&](double t0, double t1, std::vector<double>& c) {
+    c.assign(order, 0.0);
+    double fact = 1.0;
+    for (int i = 0; i < order; ++i) {
+      fact *= (i + 1);
+      c[i] = (std::pow(t1, i + 1) - std::pow(t0, i + 1)) / fact;
+    }
+  };
+  for (int m = 0; m < macroSteps; ++m)
+    for (std::int64_t k = 0; k < fine; ++k) {
+#pragma omp parallel for schedule(dynamic, 64)
+      for (std::int64_t e = 0; e < n; ++e) {
+        const int c = clusterIds[e];
+        const std::int64_t span = std::int64_t(1) << c;
+        if (k % span != 0) continue;
+        std::vector<double> coeffs, I(q), d(dsize);
+        integrate(0.0, dt0 * span, coeffs);
+        oracle_ader(order, gm, 1, coeffs.data(), dofs + e * q, star + e * 243, I.data(), d.data());
+        oracle_local(order, gm, 1, I.data(), star + e * 243, aplus + e * 324, faceTypes ? faceTypes + 4 * e : nullptr,
+                     dofs + e * q);
+        const std::uint16_t s = setups[e];
+        const bool reset = ((k / span) % 2) == 0;
+        if (s & (1u << 8)) {
+          double* b = buffers.data() + e * q;
+          const bool add = (s & (1u << 10)) && !reset;
+          for (std::int64_t i = 0; i < q; ++i) b[i] = add ? b[i] + I[i] : I[i];
+        }
+        if (s & (1u << 9)) std::memcpy(derivs.data() + e * dsize, d.data(), dsize * sizeof(double));
+      }
+#pragma omp parallel for schedule(dynamic, 64)
+      for (std::int64_t e = 0; e < n; ++e) {
+        const int c = clusterIds[e];
+        const std::int64_t span = std::int64_t(1) << c;
+        if ((k + 1) % span != 0) continue;
+        const std::int64_t start = k + 1 - span;
+        const std::uint16_t s = setups[e];
+        std::vector<double> faceI(4 * q, 0.0), coeffs, tmp(q);
+        for (int f = 0; f < 4; ++f) {
+          if (faceTypes && faceTypes[4 * e + f] != 0) continue;
+          const std::int64_t nb = neighbors[4 * e + f];
+          if (s & (1u << f)) {
+            const std::int64_t nspan = std::int64_t(1) << clusterIds[nb];
+            const std::int64_t nStart = (start / nspan) * nspan;
+            const double sub = dt0 * static_cast<double>(start - nStart);
+            integrate(sub, sub + dt0 * span, coeffs);
+            oracle_taylor(order, 1, coeffs.data(), derivs.data() + nb * dsize, tmp.data());
+            std::memcpy(faceI.data() + f * q, tmp.data(), q * sizeof(double));
+          } else {
+            std::memcpy(faceI.data() + f * q, buffers.data() + nb * q, q * sizeof(double));
+          }
+        }
+        const std::int32_t local[4] = {0, 1, 2, 3};
+        oracle_neighbor(order, gm, 1, faceI.data(), local, faceRelations + 8 * e, aminus + 324 * e,
+                        faceTypes ? faceTypes + 4 * e : nullptr, dofs + e * q);
+      }
+    }
+  return 0;
+}
+
 }  // extern "C"

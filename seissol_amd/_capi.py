@@ -17,6 +17,9 @@ SDG_FP32 = 1
 SDG_BF16X3 = 2
 SDG_BF16X6 = 3
 SDG_F16X3 = 4
+SDG_LTS_HAS_BUFFERS = 1 << 8
+SDG_LTS_HAS_DERIVATIVES = 1 << 9
+SDG_LTS_ACCUMULATE_BUFFERS = 1 << 10
 SDG_FACE_REGULAR = 0
 SDG_FACE_DYNAMIC_RUPTURE = 1
 
@@ -43,6 +46,7 @@ class Layer(C.Structure):
         ("face_relations", P),
         ("face_types", P),
         ("face_traces", P),
+        ("lts_setup", P),
     ]
 
 
@@ -126,6 +130,13 @@ SIGNATURES = {
     "sdg_mlp_adam": (I32, [P, P, P, P, I32, C.c_float, P]),
     "sdg_mlp_forward": (I32, [P, P, I64, I32, P, P]),
     "sdg_performance_estimate": (I32, [I32, I32, I32, C.POINTER(WorkEstimate)]),
+    "sdg_derive_lts_setups": (I32, [I64, P, P, P, P]),
+    "sdg_compute_local_integration": (I32, [P, P, D, C.POINTER(Layer), I32, P]),
+    "sdg_compute_neighboring_integration": (I32, [P, P, P, C.POINTER(Layer), P]),
+    "sdg_lts_create": (I32, [P, P, C.POINTER(Layer), I32, P, D]),
+    "sdg_lts_advance": (I32, [P, I32, P]),
+    "sdg_lts_graph_advance": (I32, [P, I32, P]),
+    "sdg_lts_destroy": (I32, [P]),
 }
 
 _LIB = None

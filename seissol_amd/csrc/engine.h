@@ -62,6 +62,10 @@ int launchLocal(const EngineState& e, const sdg_layer* layer, hipStream_t stream
 int launchNeighbor(const EngineState& e, const sdg_layer* layer, hipStream_t stream);
 int launchTaylor(const EngineState& e, const double* coeffs, const void* derivatives, void* out, int64_t n,
                  hipStream_t stream);
+int launchLocalIntegration(const EngineState& e, const double* coeffs, const sdg_layer* layer, int resetBuffers,
+                           hipStream_t stream);
+int launchNeighboringIntegration(const EngineState& e, const double* timeCoeffs, const double* subtimeCoeffs,
+                                 const sdg_layer* layer, hipStream_t stream);
 // kernels_mfma.hip
 int64_t mfmaFragmentCount(int order);
 int launchLocalMfma(const EngineState& e, const double* coeffs, const sdg_layer* layer, bool writeIntegrated,

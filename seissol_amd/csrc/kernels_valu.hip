@@ -60,6 +60,21 @@ struct Coeffs {
   real c[O];
 };
 
+// local time stepping of the predictor (ader + local integral)
+struct LtsArgs {
+  const uint16_t* setup;  // [n] LTS bits (SDG_LTS_*) or nullptr: global time stepping
+  int reset;              // accumulating cells store their buffer instead of adding to it
+};
+
+// local time stepping of the neighbour integral
+template <int O, typename real>
+struct NeighborLts {
+  const uint16_t* setup;  // [n] LTS bits or nullptr: every neighbour provides a buffer
+  const real* deriv;      // derivatives of all cells, [*][dsize], indexed like the buffers
+  int64_t dsize;
+  Coeffs<O, real> gts, sub;  // time-basis coefficients: GTS relation, sub-interval of a larger step
+};
+
 // Gather the 24 non-zeros of the three star matrices of the elements of the
 // block into LDS (compact order: p-major, pattern order).
 template <typename real>
@@ -141,7 +156,7 @@ __global__ __launch_bounds__(256) void kAder(int64_t n, const real* __restrict__
                                              real* __restrict__ dofs, real* __restrict__ integ,
                                              real* __restrict__ deriv, int64_t dsize,
                                              const real* __restrict__ star, const real* __restrict__ aplus,
-                                             const uint8_t* __restrict__ faceTypes) {
+                                             const uint8_t* __restrict__ faceTypes, LtsArgs lts) {
   using S = Shape<O>;
   constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
   __shared__ real sA[S::EPB * S::QS];
@@ -157,6 +172,12 @@ __global__ __launch_bounds__(256) void kAder(int64_t n, const real* __restrict__
   const bool laneOk = lane < S::EPW * TPE;
   const int64_t e = e0 + slot;
   const bool active = laneOk && e < n;
+  // LTS (TimeCluster.cpp:380-434): which of buffer / derivatives this cell
+  // provides, and whether its buffer accumulates over the sub-steps
+  const unsigned setup = (lts.setup != nullptr && active) ? lts.setup[e] : 0u;
+  const bool storeDeriv = deriv != nullptr && active && (lts.setup == nullptr || (setup & SDG_LTS_HAS_DERIVATIVES));
+  const bool storeBuffer = active && (lts.setup == nullptr || (setup & SDG_LTS_HAS_BUFFERS));
+  const bool accumulate = lts.setup != nullptr && !lts.reset && (setup & SDG_LTS_ACCUMULATE_BUFFERS);
 
   // stage Q as [row][quantity]
   for (int idx = threadIdx.x; idx < S::EPB * S::QS; idx += blockDim.x) {
@@ -183,7 +204,7 @@ __global__ __launch_bounds__(256) void kAder(int64_t n, const real* __restrict__
       iacc[r][p] = cf.c[0] * qreg[r][p];
     }
   }
-  if (deriv != nullptr && active) {
+  if (storeDeriv) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int row = t + r * TPE;
@@ -219,7 +240,7 @@ __global__ __launch_bounds__(256) void kAder(int64_t n, const real* __restrict__
           dst[row * 9 + p] = dq[r][p];
           iacc[r][p] += cf.c[d] * dq[r][p];
         }
-        if (deriv != nullptr && active)
+        if (storeDeriv)
           for (int p = 0; p < 9; ++p) deriv[e * dsize + doff + p * m + row] = dq[r][p];
       }
     }
@@ -228,13 +249,16 @@ __global__ __launch_bounds__(256) void kAder(int64_t n, const real* __restrict__
     nIn = m;
   }
 
-  if (active) {
+  if (storeBuffer) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int row = t + r * TPE;
       if (row < B)
 #pragma unroll
-        for (int p = 0; p < 9; ++p) integ[e * S::QS + p * B + row] = iacc[r][p];
+        for (int p = 0; p < 9; ++p) {
+          real* b = integ + e * S::QS + p * B + row;
+          *b = accumulate ? *b + iacc[r][p] : iacc[r][p];
+        }
     }
   }
   if constexpr (WithLocal) {
@@ -420,7 +444,7 @@ __global__ __launch_bounds__(256) void kNeighbor(int64_t n, const real* __restri
                                                  const int32_t* __restrict__ neighbors,
                                                  const uint8_t* __restrict__ faceRel,
                                                  const real* __restrict__ aminus,
-                                                 const uint8_t* __restrict__ faceTypes) {
+                                                 const uint8_t* __restrict__ faceTypes, NeighborLts<O, real> lts) {
   using S = Shape<O>;
   constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
   __shared__ real sA[S::EPB * S::QS];
@@ -446,10 +470,28 @@ __global__ __launch_bounds__(256) void kNeighbor(int64_t n, const real* __restri
     // clamp so malformed relations can never index outside the matrix table
     const int j = regular ? min(static_cast<int>(faceRel[(e * 4 + f) * 2 + 0]), 3) : 0;
     const int h = regular ? min(static_cast<int>(faceRel[(e * 4 + f) * 2 + 1]), 2) : 0;
+    // the neighbour's time integral: its buffer, or its derivatives integrated
+    // over this step (TimeCommon.cpp:40-87): GTS relation -> time coefficients,
+    // else the sub-interval coefficients
+    const unsigned setup = (lts.setup != nullptr && regular) ? lts.setup[e] : 0u;
+    const bool fromDerivatives = (setup >> f) & 1u;
+    const Coeffs<O, real>& cf = ((setup >> (4 + f)) & 1u) ? lts.gts : lts.sub;
     if (laneOk) {
       for (int idx = t; idx < S::QS; idx += TPE) {
         const int p = idx / B, l = idx % B;
-        xa[l * 9 + p] = regular ? integ[nb * S::QS + idx] : real(0);
+        real v = real(0);
+        if (regular && !fromDerivatives) {
+          v = integ[nb * S::QS + idx];
+        } else if (regular) {
+          int64_t off = nb * lts.dsize;
+#pragma unroll
+          for (int i = 0; i < O; ++i) {
+            const int m = basis3(O - i);
+            if (l < m) v += cf.c[i] * lts.deriv[off + p * m + l];
+            off += 9 * m;
+          }
+        }
+        xa[l * 9 + p] = v;
       }
     }
     __syncthreads();
@@ -546,23 +588,25 @@ int64_t derivativesSize(int order) {
 }
 
 template <typename real, int O>
-int aderImpl(const EngineState& st, const double* coeffs, const sdg_layer* L, hipStream_t stream, bool withLocal) {
+int aderImpl(const EngineState& st, const double* coeffs, const sdg_layer* L, hipStream_t stream, bool withLocal,
+             int reset) {
   using S = Shape<O>;
   const int64_t n = L->num_elements;
   if (n == 0) return SDG_OK;
   const dim3 grid(static_cast<unsigned>((n + S::EPB - 1) / S::EPB));
   const auto cf = makeCoeffs<real, O>(coeffs);
   const auto* mats = static_cast<const real*>(st.matrices);
+  const LtsArgs lts{L->lts_setup, reset};
   if (withLocal)
     kAder<real, O, true><<<grid, S::THREADS, 0, stream>>>(
         n, mats, cf, static_cast<real*>(L->dofs), static_cast<real*>(L->time_integrated),
         static_cast<real*>(L->derivatives), derivativesSize(O), static_cast<const real*>(L->star),
-        static_cast<const real*>(L->aplus), L->face_types);
+        static_cast<const real*>(L->aplus), L->face_types, lts);
   else
     kAder<real, O, false><<<grid, S::THREADS, 0, stream>>>(
         n, mats, cf, static_cast<real*>(L->dofs), static_cast<real*>(L->time_integrated),
         static_cast<real*>(L->derivatives), derivativesSize(O), static_cast<const real*>(L->star),
-        static_cast<const real*>(L->aplus), L->face_types);
+        static_cast<const real*>(L->aplus), L->face_types, lts);
   SDG_HIP_CHECK(hipGetLastError());
   return SDG_OK;
 }
@@ -582,15 +626,24 @@ int localImpl(const EngineState& st, const sdg_layer* L, hipStream_t stream) {
 }
 
 template <typename real, int O>
-int neighborImpl(const EngineState& st, const sdg_layer* L, hipStream_t stream) {
+int neighborImpl(const EngineState& st, const sdg_layer* L, hipStream_t stream, const double* gtsCoeffs,
+                 const double* subCoeffs) {
   using S = Shape<O>;
   const int64_t n = L->num_elements;
   if (n == 0) return SDG_OK;
   const dim3 grid(static_cast<unsigned>((n + S::EPB - 1) / S::EPB));
+  NeighborLts<O, real> lts{};
+  if (gtsCoeffs != nullptr) {
+    lts.setup = L->lts_setup;
+    lts.deriv = static_cast<const real*>(L->derivatives);
+    lts.dsize = derivativesSize(O);
+    lts.gts = makeCoeffs<real, O>(gtsCoeffs);
+    lts.sub = makeCoeffs<real, O>(subCoeffs);
+  }
   kNeighbor<real, O><<<grid, S::THREADS, 0, stream>>>(
       n, static_cast<const real*>(st.matrices), static_cast<real*>(L->dofs),
       static_cast<const real*>(L->time_integrated), L->face_neighbors, L->face_relations,
-      static_cast<const real*>(L->aminus), L->face_types);
+      static_cast<const real*>(L->aminus), L->face_types, lts);
   SDG_HIP_CHECK(hipGetLastError());
   return SDG_OK;
 }
@@ -623,8 +676,8 @@ int dispatch(const EngineState& st, Args&&... args) {
 
 template <typename real, int O>
 struct AderOp {
-  static int run(const EngineState& st, const double* c, const sdg_layer* L, hipStream_t s, bool wl) {
-    return aderImpl<real, O>(st, c, L, s, wl);
+  static int run(const EngineState& st, const double* c, const sdg_layer* L, hipStream_t s, bool wl, int reset) {
+    return aderImpl<real, O>(st, c, L, s, wl, reset);
   }
 };
 template <typename real, int O>
@@ -633,8 +686,8 @@ struct LocalOp {
 };
 template <typename real, int O>
 struct NeighborOp {
-  static int run(const EngineState& st, const sdg_layer* L, hipStream_t s) {
-    return neighborImpl<real, O>(st, L, s);
+  static int run(const EngineState& st, const sdg_layer* L, hipStream_t s, const double* g, const double* sub) {
+    return neighborImpl<real, O>(st, L, s, g, sub);
   }
 };
 template <typename real, int O>
@@ -660,16 +713,25 @@ DeviceMatrixLayout deviceMatrixLayout(int order) {
 }
 
 int launchAder(const EngineState& e, const double* coeffs, const sdg_layer* layer, hipStream_t stream) {
-  return dispatch<AderOp>(e, coeffs, layer, stream, false);
+  return dispatch<AderOp>(e, coeffs, layer, stream, false, 1);
 }
 int launchAderLocal(const EngineState& e, const double* coeffs, const sdg_layer* layer, hipStream_t stream) {
-  return dispatch<AderOp>(e, coeffs, layer, stream, true);
+  return dispatch<AderOp>(e, coeffs, layer, stream, true, 1);
+}
+int launchLocalIntegration(const EngineState& e, const double* coeffs, const sdg_layer* layer, int resetBuffers,
+                           hipStream_t stream) {
+  return dispatch<AderOp>(e, coeffs, layer, stream, true, resetBuffers);
+}
+int launchNeighboringIntegration(const EngineState& e, const double* timeCoeffs, const double* subtimeCoeffs,
+                                 const sdg_layer* layer, hipStream_t stream) {
+  return dispatch<NeighborOp>(e, layer, stream, timeCoeffs, subtimeCoeffs);
 }
 int launchLocal(const EngineState& e, const sdg_layer* layer, hipStream_t stream) {
   return dispatch<LocalOp>(e, layer, stream);
 }
 int launchNeighbor(const EngineState& e, const sdg_layer* layer, hipStream_t stream) {
-  return dispatch<NeighborOp>(e, layer, stream);
+  return dispatch<NeighborOp>(e, layer, stream, static_cast<const double*>(nullptr),
+                              static_cast<const double*>(nullptr));
 }
 int launchTaylor(const EngineState& e, const double* coeffs, const void* derivatives, void* out, int64_t n,
                  hipStream_t stream) {

@@ -1,0 +1,286 @@
+// Local time stepping (rate 2): the per-cell LTS setup, the cluster-wise
+// local and neighbour integration, and the macro step of all clusters,
+// enqueued directly or replayed from a captured hipGraph.
+//
+// Clusters: cluster c advances with dt_c = dt0 * 2^c; its cells are the
+// contiguous range [start[c], start[c+1]) of one layer whose arrays hold all
+// clusters, so neighbour indices are global.  One macro step (dt of the
+// slowest cluster) runs, per cluster step,
+//   predict(c, t)  - ader + local integral (TimeCluster::computeLocalIntegration,
+//                    TimeCluster.cpp:358-434): buffer (stored or accumulated
+//                    over the two sub-steps a slower neighbour sees) and
+//                    derivatives of the cells that provide them,
+//   step(c-1, t), step(c-1, t + dt_{c-1})  - the two steps of the faster cluster,
+//   correct(c, t)  - neighbour integral (TimeCluster::computeNeighboringIntegration,
+//                    TimeCluster.cpp:540-628) from buffers or from derivatives
+//                    over [t, t + dt_c] (TimeCommon.cpp:40-87).
+// A faster cluster reads a slower neighbour's derivatives, expanded at the
+// start T of that neighbour's step, over [t - T, t - T + dt_c] (the
+// subTimeStart of TimeCluster.cpp:799); the slower cluster reads the faster
+// cluster's buffer summed over its two steps.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "engine.h"
+
+using namespace sdg;
+
+namespace {
+
+constexpr uint16_t kNHD0 = 1u << 0;  // neighbour on face 0 provides derivatives
+constexpr uint16_t kGTS0 = 1u << 4;  // neighbour on face 0 is in the same cluster
+constexpr uint16_t kBuffers = SDG_LTS_HAS_BUFFERS;
+constexpr uint16_t kDerivatives = SDG_LTS_HAS_DERIVATIVES;
+constexpr uint16_t kAccumulate = SDG_LTS_ACCUMULATE_BUFFERS;
+
+int64_t derivativesReals(int order) {
+  int64_t s = 0;
+  for (int i = 0; i < order; ++i) s += 9LL * (order - i) * (order - i + 1) * (order - i + 2) / 6;
+  return s;
+}
+
+bool ltsLayerOk(const sdg_layer* L, bool neighbors, const char* who) {
+  if (L == nullptr || L->num_elements < 0) {
+    setError(std::string(who) + ": invalid layer");
+    return false;
+  }
+  if (L->num_elements == 0) return true;
+  if (L->dofs == nullptr || L->time_integrated == nullptr || L->star == nullptr || L->aplus == nullptr ||
+      L->lts_setup == nullptr || L->derivatives == nullptr) {
+    setError(std::string(who) + ": dofs, buffers, derivatives, star, aplus and lts_setup are required");
+    return false;
+  }
+  if (neighbors && (L->face_neighbors == nullptr || L->face_relations == nullptr || L->aminus == nullptr)) {
+    setError(std::string(who) + ": neighbour data missing");
+    return false;
+  }
+  return true;
+}
+
+bool vectorArithmetic(const sdg_engine* e, const char* who) {
+  if (e->s.arithmetic != SDG_FP64 && e->s.arithmetic != SDG_FP32) {
+    setError(std::string(who) + ": local time stepping runs on the FP64 and FP32 engines");
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" int sdg_derive_lts_setups(int64_t n, const int32_t* cluster_ids, const int32_t* neighbors,
+                                     const uint8_t* face_types, uint16_t* setups) {
+  if (n < 0 || (n > 0 && (cluster_ids == nullptr || neighbors == nullptr || setups == nullptr))) {
+    setError("sdg_derive_lts_setups: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  for (int64_t e = 0; e < n * 4; ++e)
+    if (neighbors[e] < 0 || neighbors[e] >= n) {
+      setError("sdg_derive_lts_setups: neighbour index out of range");
+      return SDG_ERR_INVALID;
+    }
+  // getLtsSetup (LtsSetup.cpp:97-182)
+  for (int64_t e = 0; e < n; ++e) {
+    uint16_t s = 0;
+    const int own = cluster_ids[e];
+    for (int f = 0; f < 4; ++f) {
+      const uint8_t type = face_types != nullptr ? face_types[e * 4 + f] : SDG_FACE_REGULAR;
+      if (type == SDG_FACE_DYNAMIC_RUPTURE) {
+        s |= (kNHD0 << f) | (kGTS0 << f) | kDerivatives;
+        continue;
+      }
+      const int nc = cluster_ids[neighbors[e * 4 + f]];
+      if (nc > own + 1 || own > nc + 1) {
+        // the neighbour integral takes the sub-interval of the next cluster only
+        setError("sdg_derive_lts_setups: face neighbours must be at most one cluster apart");
+        return SDG_ERR_INVALID;
+      }
+      if (own < nc) s |= (kNHD0 << f) | kAccumulate;
+      else if (own == nc) s |= (kGTS0 << f);
+      s |= own > nc ? kDerivatives : kBuffers;
+    }
+    if ((s & kAccumulate) && (s & (0xfu << 4))) s |= kDerivatives;
+    setups[e] = s;
+  }
+  // normalizeLtsSetup (LtsSetup.cpp:205-217): a same-cluster neighbour whose
+  // buffer accumulates over several steps provides derivatives instead
+  std::vector<uint16_t> first(setups, setups + n);
+  for (int64_t e = 0; e < n; ++e)
+    for (int f = 0; f < 4; ++f)
+      if ((first[e] & (kGTS0 << f)) && (first[neighbors[e * 4 + f]] & kAccumulate)) setups[e] |= (kNHD0 << f);
+  return SDG_OK;
+}
+
+extern "C" int sdg_compute_local_integration(sdg_engine* engine, const double* coeffs, double /*time_step_width*/,
+                                             const sdg_layer* layer, int reset_buffers, void* stream) {
+  if (engine == nullptr || coeffs == nullptr) {
+    setError("sdg_compute_local_integration: NULL argument");
+    return SDG_ERR_INVALID;
+  }
+  if (!vectorArithmetic(engine, "sdg_compute_local_integration") ||
+      !ltsLayerOk(layer, false, "sdg_compute_local_integration"))
+    return SDG_ERR_INVALID;
+  return launchLocalIntegration(engine->s, coeffs, layer, reset_buffers != 0, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int sdg_compute_neighboring_integration(sdg_engine* engine, const double* time_coeffs,
+                                                   const double* subtime_coeffs, const sdg_layer* layer,
+                                                   void* stream) {
+  if (engine == nullptr || time_coeffs == nullptr || subtime_coeffs == nullptr) {
+    setError("sdg_compute_neighboring_integration: NULL argument");
+    return SDG_ERR_INVALID;
+  }
+  if (!vectorArithmetic(engine, "sdg_compute_neighboring_integration") ||
+      !ltsLayerOk(layer, true, "sdg_compute_neighboring_integration"))
+    return SDG_ERR_INVALID;
+  return launchNeighboringIntegration(engine->s, time_coeffs, subtime_coeffs, layer,
+                                      static_cast<hipStream_t>(stream));
+}
+
+struct sdg_lts {
+  sdg_engine* engine = nullptr;
+  sdg_layer layer{};
+  std::vector<int64_t> start;
+  double dt0 = 0.0;
+  hipStream_t captureStream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// view of cluster c: own arrays offset to the cluster, neighbour sources global
+sdg_layer clusterView(const sdg_lts& L, int c, bool neighborPass) {
+  const int order = L.engine->s.order;
+  const int64_t rs = L.engine->s.arithmetic == SDG_FP64 ? 8 : 4;
+  const int64_t q = 9LL * order * (order + 1) * (order + 2) / 6;
+  const int64_t off = L.start[c];
+  const sdg_layer& g = L.layer;
+  sdg_layer v = g;
+  auto shift = [&](const void* p, int64_t bytes) -> char* {
+    return p == nullptr ? nullptr : const_cast<char*>(static_cast<const char*>(p)) + off * bytes;
+  };
+  v.num_elements = L.start[c + 1] - L.start[c];
+  v.dofs = shift(g.dofs, q * rs);
+  v.star = shift(g.star, 243 * rs);
+  v.aplus = shift(g.aplus, 324 * rs);
+  v.aminus = shift(g.aminus, 324 * rs);
+  v.face_neighbors = reinterpret_cast<const int32_t*>(shift(g.face_neighbors, 16));
+  v.face_relations = reinterpret_cast<const uint8_t*>(shift(g.face_relations, 8));
+  v.face_types = reinterpret_cast<const uint8_t*>(shift(g.face_types, 4));
+  v.lts_setup = reinterpret_cast<const uint16_t*>(shift(g.lts_setup, 2));
+  v.face_traces = nullptr;
+  if (!neighborPass) {
+    v.time_integrated = shift(g.time_integrated, q * rs);
+    v.derivatives = shift(g.derivatives, derivativesReals(order) * rs);
+  }
+  return v;
+}
+
+// one step of cluster c starting at time k * dt0, with its faster clusters nested
+int stepCluster(const sdg_lts& L, int c, int64_t k, hipStream_t s) {
+  const int order = L.engine->s.order;
+  const int C = static_cast<int>(L.start.size()) - 1;
+  const double dtc = L.dt0 * static_cast<double>(int64_t(1) << c);
+  const int64_t stepIndex = k >> c;  // steps of cluster c since the macro step began
+  std::vector<double> timeCoeffs(order), subCoeffs(order);
+  sdg_monomial_integrate(order, 0.0, dtc, timeCoeffs.data());
+  const sdg_layer own = clusterView(L, c, false);
+  int rc = SDG_OK;
+  if (own.num_elements > 0) rc = launchLocalIntegration(L.engine->s, timeCoeffs.data(), &own, (stepIndex & 1) == 0, s);
+  if (rc != SDG_OK) return rc;
+  if (c > 0) {
+    if ((rc = stepCluster(L, c - 1, k, s)) != SDG_OK) return rc;
+    if ((rc = stepCluster(L, c - 1, k + (int64_t(1) << (c - 1)), s)) != SDG_OK) return rc;
+  }
+  // sub-interval of the slower cluster's step
+  if (c + 1 < C) {
+    const int64_t slowStart = (k >> (c + 1)) << (c + 1);
+    const double sub = L.dt0 * static_cast<double>(k - slowStart);
+    sdg_monomial_integrate(order, sub, sub + dtc, subCoeffs.data());
+  } else {
+    subCoeffs = timeCoeffs;
+  }
+  const sdg_layer nbv = clusterView(L, c, true);
+  if (nbv.num_elements > 0) rc = launchNeighboringIntegration(L.engine->s, timeCoeffs.data(), subCoeffs.data(), &nbv, s);
+  return rc;
+}
+
+int macroStep(const sdg_lts& L, hipStream_t s) {
+  return stepCluster(L, static_cast<int>(L.start.size()) - 2, 0, s);
+}
+
+}  // namespace
+
+extern "C" int sdg_lts_create(sdg_lts** out, sdg_engine* engine, const sdg_layer* layer, int num_clusters,
+                              const int64_t* cluster_start, double dt0) {
+  if (out == nullptr || engine == nullptr || cluster_start == nullptr || num_clusters < 1 || num_clusters > 16 ||
+      !(dt0 > 0.0)) {
+    setError("sdg_lts_create: invalid arguments (1..16 clusters, dt0 > 0)");
+    return SDG_ERR_INVALID;
+  }
+  if (!vectorArithmetic(engine, "sdg_lts_create") || !ltsLayerOk(layer, true, "sdg_lts_create"))
+    return SDG_ERR_INVALID;
+  if (cluster_start[0] != 0 || cluster_start[num_clusters] != layer->num_elements) {
+    setError("sdg_lts_create: cluster_start must run from 0 to num_elements");
+    return SDG_ERR_INVALID;
+  }
+  for (int c = 0; c < num_clusters; ++c)
+    if (cluster_start[c + 1] < cluster_start[c]) {
+      setError("sdg_lts_create: cluster_start must not decrease");
+      return SDG_ERR_INVALID;
+    }
+  auto* L = new sdg_lts;
+  L->engine = engine;
+  L->layer = *layer;
+  L->start.assign(cluster_start, cluster_start + num_clusters + 1);
+  L->dt0 = dt0;
+  *out = L;
+  return SDG_OK;
+}
+
+extern "C" int sdg_lts_advance(sdg_lts* lts, int macro_steps, void* stream) {
+  if (lts == nullptr || macro_steps < 0) {
+    setError("sdg_lts_advance: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  for (int m = 0; m < macro_steps; ++m) {
+    const int rc = macroStep(*lts, static_cast<hipStream_t>(stream));
+    if (rc != SDG_OK) return rc;
+  }
+  return SDG_OK;
+}
+
+extern "C" int sdg_lts_graph_advance(sdg_lts* lts, int macro_steps, void* stream) {
+  if (lts == nullptr || macro_steps < 0) {
+    setError("sdg_lts_graph_advance: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  if (lts->exec == nullptr) {
+    // capture one macro step on a private stream (default priority)
+    if (lts->captureStream == nullptr) SDG_HIP_CHECK(hipStreamCreateWithFlags(&lts->captureStream, hipStreamNonBlocking));
+    SDG_HIP_CHECK(hipStreamBeginCapture(lts->captureStream, hipStreamCaptureModeThreadLocal));
+    const int rc = macroStep(*lts, lts->captureStream);
+    hipGraph_t g = nullptr;
+    const hipError_t end = hipStreamEndCapture(lts->captureStream, &g);
+    if (rc != SDG_OK) {
+      if (g != nullptr) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    SDG_HIP_CHECK(end);
+    lts->graph = g;
+    SDG_HIP_CHECK(hipGraphInstantiate(&lts->exec, lts->graph, nullptr, nullptr, 0));
+  }
+  for (int m = 0; m < macro_steps; ++m) SDG_HIP_CHECK(hipGraphLaunch(lts->exec, static_cast<hipStream_t>(stream)));
+  return SDG_OK;
+}
+
+extern "C" int sdg_lts_destroy(sdg_lts* lts) {
+  if (lts == nullptr) return SDG_OK;
+  if (lts->exec != nullptr) (void)hipGraphExecDestroy(lts->exec);
+  if (lts->graph != nullptr) (void)hipGraphDestroy(lts->graph);
+  if (lts->captureStream != nullptr) (void)hipStreamDestroy(lts->captureStream);
+  delete lts;
+  return SDG_OK;
+}

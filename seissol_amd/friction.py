@@ -270,21 +270,30 @@ class WarmStartTrainer:
         check(lib().sdg_mlp_adam(ptr(self.params), ptr(self.grads), ptr(self.m), ptr(self.v), self.step, lr, None),
               "sdg_mlp_adam")
 
-    def fit(self, samples, epochs: int = 4, batch: int = 4096, lr: float = 3e-3, seed: int = 0) -> list:
+    def fit(self, samples, steps: int = 3000, batch: int = 1024, lr: float = 3e-3, seed: int = 0,
+            balance: bool = True) -> list:
+        """`steps` Adam steps on random batches, learning rate decaying to lr/20.
+
+        With `balance`, the solves whose target is non-negligible (|target| >
+        0.01, the closed-form estimate is off) are mixed 1:1 with the others,
+        which would otherwise dominate the loss with near-zero targets."""
         import torch
-        self.set_normalisation(samples)
         g = torch.Generator(device="cuda").manual_seed(seed)
+        if balance:
+            off = samples[:, 6].abs() > 0.01
+            hard, easy = samples[off], samples[~off]
+            if 0 < hard.shape[0] < easy.shape[0]:
+                pick = torch.randint(0, easy.shape[0], (hard.shape[0],), generator=g, device="cuda")
+                samples = torch.cat([hard, easy[pick]]).contiguous()
+        self.set_normalisation(samples)
         n = samples.shape[0]
         history = []
-        for ep in range(epochs):
-            perm = torch.randperm(n, generator=g, device="cuda")
-            total = 0.0
-            for i in range(0, n - batch + 1, batch):
-                b = samples[perm[i:i + batch]].contiguous()
-                self.loss_grad(b)
-                self.adam(lr * (0.5 ** ep))
-                total += float(self.loss.item()) if i % (batch * 64) == 0 else 0.0
-            history.append(total)
+        for k in range(steps):
+            idx = torch.randint(0, n, (min(batch, n),), generator=g, device="cuda")
+            self.loss_grad(samples[idx].contiguous())
+            self.adam(lr * (0.05 ** (k / max(1, steps - 1))))
+            if k % 500 == 0 or k == steps - 1:
+                history.append(float(self.loss.item()))
         torch.cuda.synchronize()
         return history
 

@@ -1,0 +1,83 @@
+"""Local time stepping on the GPU against the oracle, and its hipGraph replay."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import rel_l2
+from seissol_amd import _capi
+from seissol_amd.kernels import Engine, global_matrices
+from seissol_amd.lts import DeviceLtsLayer, LocalTimeStepping, make_lts_setup
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _oracle():
+    oracle.build()
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _oracle_q(lts, order, steps, layer):
+    s = lts.setup
+    gm = global_matrices(order)["flat"]
+    return oracle.lts(order, gm, steps, lts.num_clusters, lts.cluster_ids, lts.dt0, lts.lts_setups,
+                      _np(layer.star64), _np(layer.aplus64), _np(layer.aminus64), s.neighbors, s.face_relations,
+                      s.dofs)
+
+
+@pytest.mark.parametrize("order,fractions,arith,tol", [
+    (4, (0.5, 0.3, 0.2), "fp64", 1e-11),
+    (5, (0.6, 0.25, 0.1, 0.05), "fp64", 1e-11),
+    (3, (0.4, 0.35, 0.25), "fp32", 2e-5),
+])
+def test_lts_matches_oracle(gpu, order, fractions, arith, tol):
+    import torch
+    dims = (20, 6, 6) if len(fractions) == 4 else (10, 6, 6)
+    lts = make_lts_setup(order, fractions=fractions, dims=dims, seed=3)
+    print(f"lts order={order} clusters={lts.num_clusters} n={lts.setup.n} {arith}", flush=True)
+    assert lts.num_clusters == len(fractions) and np.all(np.diff(lts.cluster_start) > 0)
+    eng = Engine(order, arith)
+    layer = DeviceLtsLayer(eng, lts)
+    run = LocalTimeStepping(eng, layer)
+    run.advance(2)
+    torch.cuda.synchronize()
+    q_ref = _oracle_q(lts, order, 2, layer)
+    err = rel_l2(_np(layer.dofs).astype(np.float64), q_ref)
+    print(f"rel L2 vs oracle after 2 macro steps: {err:.2e}")
+    assert err < tol
+
+
+def test_lts_graph_replay_equals_eager(gpu):
+    import torch
+    order = 4
+    lts = make_lts_setup(order, fractions=(0.5, 0.3, 0.2), dims=(10, 6, 6), seed=8)
+    eng = Engine(order, "fp64")
+    eager_layer = DeviceLtsLayer(eng, lts)
+    eager = LocalTimeStepping(eng, eager_layer)
+    graph_layer = DeviceLtsLayer(eng, lts)
+    graphed = LocalTimeStepping(eng, graph_layer)
+    for step in range(3):
+        eager.advance(1)
+        graphed.advance(1, graph=True)
+        torch.cuda.synchronize()
+        # the program's own eager work between replays
+        scratch = torch.ones(1 << 16, device="cuda").cumsum(0)
+        torch.cuda.synchronize()
+        assert bool(torch.equal(eager_layer.dofs, graph_layer.dofs)), f"replay {step} differs"
+        del scratch
+
+
+def test_lts_rejects_matrix_core_engines_and_missing_setup(gpu):
+    lts = make_lts_setup(3, fractions=(0.5, 0.5), dims=(6, 4, 4), seed=1)
+    eng = Engine(3, "f16x3")
+    layer = DeviceLtsLayer(eng, lts)
+    with pytest.raises(_capi.SdgError):
+        LocalTimeStepping(eng, layer)
+    eng64 = Engine(3, "fp64")
+    layer64 = DeviceLtsLayer(eng64, lts)
+    layer64.view.lts_setup = None
+    with pytest.raises(_capi.SdgError):
+        LocalTimeStepping(eng64, layer64)
