@@ -168,8 +168,24 @@ def lts_bench(args) -> dict:
     run = LocalTimeStepping(eng, layer)
     upd = lts.updates_per_macro_step()
     out = {}
-    for mode in ("eager", "graph"):
-        graph = mode == "graph"
+    # default tiles first, then the autotuned ones
+    for label in ("default_tiles", "autotuned"):
+        if label == "autotuned":
+            t0 = time.perf_counter()
+            tiles = run.autotune(repeats=3)
+            log(f"autotuned tiles {tiles} in {time.perf_counter() - t0:.1f} s")
+        run.advance(1)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        run.advance(args.lts_steps)
+        e.record()
+        torch.cuda.synchronize()
+        ms = s.elapsed_time(e) / args.lts_steps
+        out[f"eager_{label}"] = {"ms_per_macro_step": ms, "element_updates_per_s": upd / (ms / 1e3)}
+        log(f"lts eager ({label}): {ms:.3f} ms per macro step -> {upd / (ms / 1e3) / 1e6:.2f} M element-updates/s")
+    for mode in ("graph",):
+        graph = True
         run.advance(1, graph=graph)
         torch.cuda.synchronize()
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -185,8 +201,11 @@ def lts_bench(args) -> dict:
     gts_updates = lts.setup.n * (1 << (C_ - 1))  # the same simulated time at dt0 everywhere
     return {
         "metric": "element-updates/s, rate-2 LTS clusters, ORDER 5 elastic",
-        "value": out["graph"]["element_updates_per_s"], "unit": "element-updates/s", "higher_is_better": True,
-        "modes": out, "dtype": {"fp64": "f64", "fp32": "f32"}[args.lts_form],
+        "value": out["graph"]["element_updates_per_s"],
+        "graph_vs_eager": out["eager_autotuned"]["ms_per_macro_step"] / out["graph"]["ms_per_macro_step"], "unit": "element-updates/s", "higher_is_better": True,
+        "modes": out, "tiles": {str(c): t for c, t in run.tiles().items()},
+        "tile_variants": "bit 0: one more basis row per lane; bit 1: two waves per block",
+        "dtype": {"fp64": "f64", "fp32": "f32"}[args.lts_form],
         "config": {"workload": f"ORDER {args.lts_order} elastic, {lts.setup.n} tetrahedra (periodic box "
                                f"{lts.setup.dims}), clusters {np.diff(lts.cluster_start).tolist()} with dt0 * 2^c, "
                                f"{args.lts_steps} macro steps",

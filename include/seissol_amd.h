@@ -182,6 +182,15 @@ int sdg_lts_advance(sdg_lts* lts, int macro_steps, void* stream);
 /* Same, replaying a hipGraph of one macro step captured on first use
  * (StreamRuntime::runGraph, src/Parallel/Runtime/Stream.h:163-190). */
 int sdg_lts_graph_advance(sdg_lts* lts, int macro_steps, void* stream);
+/* Tile autotuner (no reference counterpart: tile shapes are the engine's
+ * own): times the tile variants of both kernels of every cluster on the
+ * cluster's data, keeps the fastest and restores the cells' state.
+ * Allocates temporaries for the saved state; synchronises `stream`. */
+int sdg_lts_autotune(sdg_lts* lts, int repeats, void* stream);
+/* Tile variant in use for `cluster` and kernel 0 (local) / 1 (neighbour):
+ * bit 0 = one more basis row per lane, bit 1 = two waves per block. */
+int sdg_lts_tile_variant(const sdg_lts* lts, int cluster, int kernel);
+int sdg_lts_set_tile_variant(sdg_lts* lts, int cluster, int kernel, int variant);
 int sdg_lts_destroy(sdg_lts* lts);
 
 /* Proxy kernel selection (src/Proxy/Common.cpp:22-41, KernelDevice.cpp:144-165):

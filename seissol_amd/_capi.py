@@ -137,6 +137,9 @@ SIGNATURES = {
     "sdg_lts_advance": (I32, [P, I32, P]),
     "sdg_lts_graph_advance": (I32, [P, I32, P]),
     "sdg_lts_destroy": (I32, [P]),
+    "sdg_lts_autotune": (I32, [P, I32, P]),
+    "sdg_lts_tile_variant": (I32, [P, I32, I32]),
+    "sdg_lts_set_tile_variant": (I32, [P, I32, I32, I32]),
 }
 
 _LIB = None

@@ -62,10 +62,12 @@ int launchLocal(const EngineState& e, const sdg_layer* layer, hipStream_t stream
 int launchNeighbor(const EngineState& e, const sdg_layer* layer, hipStream_t stream);
 int launchTaylor(const EngineState& e, const double* coeffs, const void* derivatives, void* out, int64_t n,
                  hipStream_t stream);
+// local time stepping kernels; `variant` selects a tile shape (0..3, 0 = default)
+constexpr int kTileVariants = 4;
 int launchLocalIntegration(const EngineState& e, const double* coeffs, const sdg_layer* layer, int resetBuffers,
-                           hipStream_t stream);
+                           hipStream_t stream, int variant = 0);
 int launchNeighboringIntegration(const EngineState& e, const double* timeCoeffs, const double* subtimeCoeffs,
-                                 const sdg_layer* layer, hipStream_t stream);
+                                 const sdg_layer* layer, hipStream_t stream, int variant = 0);
 // kernels_mfma.hip
 int64_t mfmaFragmentCount(int order);
 int launchLocalMfma(const EngineState& e, const double* coeffs, const sdg_layer* layer, bool writeIntegrated,

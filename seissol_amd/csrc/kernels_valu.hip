@@ -27,14 +27,18 @@ namespace {
 constexpr int basis3(int o) { return o * (o + 1) * (o + 2) / 6; }
 constexpr int basis2(int o) { return o * (o + 1) / 2; }
 
-template <int O>
+// Tile shape: R basis rows per lane (TPE lanes per element, EPW elements per
+// wave) and WAVES waves per block.  RX = 1 takes one more row per lane than
+// the default; WV is the number of waves.  Local time stepping picks the
+// shape per cluster and kernel by timing (sdg_lts_autotune).
+template <int O, int RX = 0, int WV = 4>
 struct Shape {
   static constexpr int B = basis3(O);
   static constexpr int B2 = basis2(O);
-  static constexpr int R = B <= 10 ? 1 : (B <= 56 ? 2 : 3);
+  static constexpr int R = (B <= 10 ? 1 : (B <= 56 ? 2 : 3)) + RX;
   static constexpr int TPE = (B + R - 1) / R;
   static constexpr int EPW = 64 / TPE;
-  static constexpr int WAVES = 4;
+  static constexpr int WAVES = WV;
   static constexpr int EPB = EPW * WAVES;
   static constexpr int THREADS = 64 * WAVES;
   static constexpr int QS = 9 * B;       // reals per DOF block
@@ -151,13 +155,13 @@ __device__ inline void gemmRows(real (&acc)[J][R][9], const real* __restrict__ A
 // ---------------------------------------------------------------------------
 // ader (+ optional volume and local flux): Time.cpp:65-140 and Local.cpp:124-160
 // ---------------------------------------------------------------------------
-template <typename real, int O, bool WithLocal>
-__global__ __launch_bounds__(256) void kAder(int64_t n, const real* __restrict__ mats, Coeffs<O, real> cf,
+template <typename real, int O, bool WithLocal, int RX = 0, int WV = 4>
+__global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restrict__ mats, Coeffs<O, real> cf,
                                              real* __restrict__ dofs, real* __restrict__ integ,
                                              real* __restrict__ deriv, int64_t dsize,
                                              const real* __restrict__ star, const real* __restrict__ aplus,
                                              const uint8_t* __restrict__ faceTypes, LtsArgs lts) {
-  using S = Shape<O>;
+  using S = Shape<O, RX, WV>;
   constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
   __shared__ real sA[S::EPB * S::QS];
   __shared__ real sB[S::EPB * (S::FS > S::QS ? S::FS : S::QS)];
@@ -438,14 +442,14 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
 // ---------------------------------------------------------------------------
 // neighbour flux (Neighbor.cpp:66-92)
 // ---------------------------------------------------------------------------
-template <typename real, int O>
-__global__ __launch_bounds__(256) void kNeighbor(int64_t n, const real* __restrict__ mats, real* __restrict__ dofs,
+template <typename real, int O, int RX = 0, int WV = 4>
+__global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __restrict__ mats, real* __restrict__ dofs,
                                                  const real* __restrict__ integ,
                                                  const int32_t* __restrict__ neighbors,
                                                  const uint8_t* __restrict__ faceRel,
                                                  const real* __restrict__ aminus,
                                                  const uint8_t* __restrict__ faceTypes, NeighborLts<O, real> lts) {
-  using S = Shape<O>;
+  using S = Shape<O, RX, WV>;
   constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
   __shared__ real sA[S::EPB * S::QS];
   __shared__ real sU[S::EPB * B2 * 9];
@@ -587,10 +591,10 @@ int64_t derivativesSize(int order) {
   return s;
 }
 
-template <typename real, int O>
+template <typename real, int O, int RX = 0, int WV = 4>
 int aderImpl(const EngineState& st, const double* coeffs, const sdg_layer* L, hipStream_t stream, bool withLocal,
              int reset) {
-  using S = Shape<O>;
+  using S = Shape<O, RX, WV>;
   const int64_t n = L->num_elements;
   if (n == 0) return SDG_OK;
   const dim3 grid(static_cast<unsigned>((n + S::EPB - 1) / S::EPB));
@@ -598,12 +602,12 @@ int aderImpl(const EngineState& st, const double* coeffs, const sdg_layer* L, hi
   const auto* mats = static_cast<const real*>(st.matrices);
   const LtsArgs lts{L->lts_setup, reset};
   if (withLocal)
-    kAder<real, O, true><<<grid, S::THREADS, 0, stream>>>(
+    kAder<real, O, true, RX, WV><<<grid, S::THREADS, 0, stream>>>(
         n, mats, cf, static_cast<real*>(L->dofs), static_cast<real*>(L->time_integrated),
         static_cast<real*>(L->derivatives), derivativesSize(O), static_cast<const real*>(L->star),
         static_cast<const real*>(L->aplus), L->face_types, lts);
   else
-    kAder<real, O, false><<<grid, S::THREADS, 0, stream>>>(
+    kAder<real, O, false, RX, WV><<<grid, S::THREADS, 0, stream>>>(
         n, mats, cf, static_cast<real*>(L->dofs), static_cast<real*>(L->time_integrated),
         static_cast<real*>(L->derivatives), derivativesSize(O), static_cast<const real*>(L->star),
         static_cast<const real*>(L->aplus), L->face_types, lts);
@@ -625,10 +629,10 @@ int localImpl(const EngineState& st, const sdg_layer* L, hipStream_t stream) {
   return SDG_OK;
 }
 
-template <typename real, int O>
+template <typename real, int O, int RX = 0, int WV = 4>
 int neighborImpl(const EngineState& st, const sdg_layer* L, hipStream_t stream, const double* gtsCoeffs,
                  const double* subCoeffs) {
-  using S = Shape<O>;
+  using S = Shape<O, RX, WV>;
   const int64_t n = L->num_elements;
   if (n == 0) return SDG_OK;
   const dim3 grid(static_cast<unsigned>((n + S::EPB - 1) / S::EPB));
@@ -640,7 +644,7 @@ int neighborImpl(const EngineState& st, const sdg_layer* L, hipStream_t stream, 
     lts.gts = makeCoeffs<real, O>(gtsCoeffs);
     lts.sub = makeCoeffs<real, O>(subCoeffs);
   }
-  kNeighbor<real, O><<<grid, S::THREADS, 0, stream>>>(
+  kNeighbor<real, O, RX, WV><<<grid, S::THREADS, 0, stream>>>(
       n, static_cast<const real*>(st.matrices), static_cast<real*>(L->dofs),
       static_cast<const real*>(L->time_integrated), L->face_neighbors, L->face_relations,
       static_cast<const real*>(L->aminus), L->face_types, lts);
@@ -678,6 +682,50 @@ template <typename real, int O>
 struct AderOp {
   static int run(const EngineState& st, const double* c, const sdg_layer* L, hipStream_t s, bool wl, int reset) {
     return aderImpl<real, O>(st, c, L, s, wl, reset);
+  }
+};
+// static LDS of kAder / kNeighbor for a tile shape
+template <typename real, int O, int RX, int WV>
+constexpr int64_t tileLds() {
+  using S = Shape<O, RX, WV>;
+  const int64_t ader = S::EPB * (S::QS + (S::FS > S::QS ? S::FS : S::QS) + 3 * kStarNnz);
+  const int64_t nb = S::EPB * (S::QS + S::B2 * 9 + S::FS);
+  return static_cast<int64_t>(sizeof(real)) * (ader > nb ? ader : nb);
+}
+// waves per block of a variant: 4 unless its LDS does not fit, then 2
+template <typename real, int O, int RX, int WV>
+constexpr int fitWaves() {
+  return tileLds<real, O, RX, WV>() <= 163840 ? WV : 2;
+}
+
+// tile variants of the local time stepping kernels: bit 0 -> one more row per
+// lane, bit 1 -> two waves per block instead of four (orders 4..6)
+template <typename real, int O>
+struct LtsLocalOp {
+  static int run(const EngineState& st, const double* c, const sdg_layer* L, hipStream_t s, int reset, int v) {
+    if constexpr (O >= 4 && O <= 6) {
+      switch (v) {
+        case 1: return aderImpl<real, O, 1, fitWaves<real, O, 1, 4>()>(st, c, L, s, true, reset);
+        case 2: return aderImpl<real, O, 0, 2>(st, c, L, s, true, reset);
+        case 3: return aderImpl<real, O, 1, 2>(st, c, L, s, true, reset);
+        default: break;
+      }
+    }
+    return aderImpl<real, O>(st, c, L, s, true, reset);
+  }
+};
+template <typename real, int O>
+struct LtsNeighborOp {
+  static int run(const EngineState& st, const sdg_layer* L, hipStream_t s, const double* g, const double* sub, int v) {
+    if constexpr (O >= 4 && O <= 6) {
+      switch (v) {
+        case 1: return neighborImpl<real, O, 1, fitWaves<real, O, 1, 4>()>(st, L, s, g, sub);
+        case 2: return neighborImpl<real, O, 0, 2>(st, L, s, g, sub);
+        case 3: return neighborImpl<real, O, 1, 2>(st, L, s, g, sub);
+        default: break;
+      }
+    }
+    return neighborImpl<real, O>(st, L, s, g, sub);
   }
 };
 template <typename real, int O>
@@ -719,12 +767,12 @@ int launchAderLocal(const EngineState& e, const double* coeffs, const sdg_layer*
   return dispatch<AderOp>(e, coeffs, layer, stream, true, 1);
 }
 int launchLocalIntegration(const EngineState& e, const double* coeffs, const sdg_layer* layer, int resetBuffers,
-                           hipStream_t stream) {
-  return dispatch<AderOp>(e, coeffs, layer, stream, true, resetBuffers);
+                           hipStream_t stream, int variant) {
+  return dispatch<LtsLocalOp>(e, coeffs, layer, stream, resetBuffers, variant);
 }
 int launchNeighboringIntegration(const EngineState& e, const double* timeCoeffs, const double* subtimeCoeffs,
-                                 const sdg_layer* layer, hipStream_t stream) {
-  return dispatch<NeighborOp>(e, layer, stream, timeCoeffs, subtimeCoeffs);
+                                 const sdg_layer* layer, hipStream_t stream, int variant) {
+  return dispatch<LtsNeighborOp>(e, layer, stream, timeCoeffs, subtimeCoeffs, variant);
 }
 int launchLocal(const EngineState& e, const sdg_layer* layer, hipStream_t stream) {
   return dispatch<LocalOp>(e, layer, stream);

@@ -143,6 +143,7 @@ struct sdg_lts {
   sdg_layer layer{};
   std::vector<int64_t> start;
   double dt0 = 0.0;
+  std::vector<int> tileLocal, tileNeighbor;  // tile variant per cluster
   hipStream_t captureStream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -188,7 +189,8 @@ int stepCluster(const sdg_lts& L, int c, int64_t k, hipStream_t s) {
   sdg_monomial_integrate(order, 0.0, dtc, timeCoeffs.data());
   const sdg_layer own = clusterView(L, c, false);
   int rc = SDG_OK;
-  if (own.num_elements > 0) rc = launchLocalIntegration(L.engine->s, timeCoeffs.data(), &own, (stepIndex & 1) == 0, s);
+  if (own.num_elements > 0)
+    rc = launchLocalIntegration(L.engine->s, timeCoeffs.data(), &own, (stepIndex & 1) == 0, s, L.tileLocal[c]);
   if (rc != SDG_OK) return rc;
   if (c > 0) {
     if ((rc = stepCluster(L, c - 1, k, s)) != SDG_OK) return rc;
@@ -203,7 +205,8 @@ int stepCluster(const sdg_lts& L, int c, int64_t k, hipStream_t s) {
     subCoeffs = timeCoeffs;
   }
   const sdg_layer nbv = clusterView(L, c, true);
-  if (nbv.num_elements > 0) rc = launchNeighboringIntegration(L.engine->s, timeCoeffs.data(), subCoeffs.data(), &nbv, s);
+  if (nbv.num_elements > 0)
+    rc = launchNeighboringIntegration(L.engine->s, timeCoeffs.data(), subCoeffs.data(), &nbv, s, L.tileNeighbor[c]);
   return rc;
 }
 
@@ -236,8 +239,115 @@ extern "C" int sdg_lts_create(sdg_lts** out, sdg_engine* engine, const sdg_layer
   L->layer = *layer;
   L->start.assign(cluster_start, cluster_start + num_clusters + 1);
   L->dt0 = dt0;
+  L->tileLocal.assign(num_clusters, 0);
+  L->tileNeighbor.assign(num_clusters, 0);
   *out = L;
   return SDG_OK;
+}
+
+// Tile autotuner: times every tile variant of both kernels on every cluster
+// (`repeats` launches each, HIP events on `stream`) and keeps the fastest;
+// the cells' state is saved before and restored afterwards.
+extern "C" int sdg_lts_autotune(sdg_lts* lts, int repeats, void* stream) {
+  if (lts == nullptr || repeats < 1) {
+    setError("sdg_lts_autotune: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  const auto st = static_cast<hipStream_t>(stream);
+  const int order = lts->engine->s.order;
+  const int64_t n = lts->layer.num_elements;
+  const int64_t rs = lts->engine->s.arithmetic == SDG_FP64 ? 8 : 4;
+  const int64_t qBytes = n * 9LL * order * (order + 1) * (order + 2) / 6 * rs;
+  const int64_t dBytes = n * derivativesReals(order) * rs;
+  void* saved[3] = {nullptr, nullptr, nullptr};
+  void* live[3] = {lts->layer.dofs, lts->layer.time_integrated, lts->layer.derivatives};
+  const int64_t bytes[3] = {qBytes, qBytes, dBytes};
+  for (int i = 0; i < 3; ++i) {
+    SDG_HIP_CHECK(hipMalloc(&saved[i], bytes[i]));
+    SDG_HIP_CHECK(hipMemcpyAsync(saved[i], live[i], bytes[i], hipMemcpyDeviceToDevice, st));
+  }
+  hipEvent_t e0, e1;
+  SDG_HIP_CHECK(hipEventCreate(&e0));
+  SDG_HIP_CHECK(hipEventCreate(&e1));
+  int rc = SDG_OK;
+  const int C = static_cast<int>(lts->start.size()) - 1;
+  for (int c = 0; c < C && rc == SDG_OK; ++c) {
+    const double dtc = lts->dt0 * static_cast<double>(int64_t(1) << c);
+    std::vector<double> coeffs(order);
+    sdg_monomial_integrate(order, 0.0, dtc, coeffs.data());
+    const sdg_layer own = clusterView(*lts, c, false), nbv = clusterView(*lts, c, true);
+    if (own.num_elements == 0) continue;
+    for (int kind = 0; kind < 2 && rc == SDG_OK; ++kind) {
+      float best = 0.0f;
+      int bestV = 0;
+      for (int v = 0; v < kTileVariants && rc == SDG_OK; ++v) {
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < repeats && rc == SDG_OK; ++r)
+          rc = kind == 0 ? launchLocalIntegration(lts->engine->s, coeffs.data(), &own, 1, st, v)
+                         : launchNeighboringIntegration(lts->engine->s, coeffs.data(), coeffs.data(), &nbv, st, v);
+        (void)hipEventRecord(e1, st);
+        if (rc != SDG_OK) break;
+        const hipError_t err = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (err != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          rc = hipFail(err != hipSuccess ? err : hipErrorUnknown, "sdg_lts_autotune timing");
+          break;
+        }
+        if (v == 0 || ms < best) {
+          best = ms;
+          bestV = v;
+        }
+      }
+      (kind == 0 ? lts->tileLocal : lts->tileNeighbor)[c] = bestV;
+    }
+  }
+  for (int i = 0; i < 3; ++i) {
+    if (rc == SDG_OK) {
+      const hipError_t err = hipMemcpyAsync(live[i], saved[i], bytes[i], hipMemcpyDeviceToDevice, st);
+      if (err != hipSuccess) rc = hipFail(err, "sdg_lts_autotune restore");
+    }
+  }
+  (void)hipStreamSynchronize(st);
+  for (int i = 0; i < 3; ++i) (void)hipFree(saved[i]);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // a captured macro step used the previous tiles
+  if (lts->exec != nullptr) {
+    (void)hipGraphExecDestroy(lts->exec);
+    lts->exec = nullptr;
+  }
+  if (lts->graph != nullptr) {
+    (void)hipGraphDestroy(lts->graph);
+    lts->graph = nullptr;
+  }
+  return rc;
+}
+
+extern "C" int sdg_lts_set_tile_variant(sdg_lts* lts, int cluster, int kernel, int variant) {
+  if (lts == nullptr || cluster < 0 || cluster >= static_cast<int>(lts->tileLocal.size()) || kernel < 0 ||
+      kernel > 1 || variant < 0 || variant >= kTileVariants) {
+    setError("sdg_lts_set_tile_variant: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  (kernel == 0 ? lts->tileLocal : lts->tileNeighbor)[cluster] = variant;
+  if (lts->exec != nullptr) {
+    (void)hipGraphExecDestroy(lts->exec);
+    lts->exec = nullptr;
+  }
+  if (lts->graph != nullptr) {
+    (void)hipGraphDestroy(lts->graph);
+    lts->graph = nullptr;
+  }
+  return SDG_OK;
+}
+
+extern "C" int sdg_lts_tile_variant(const sdg_lts* lts, int cluster, int kernel) {
+  if (lts == nullptr || cluster < 0 || cluster >= static_cast<int>(lts->tileLocal.size()) || kernel < 0 ||
+      kernel > 1) {
+    setError("sdg_lts_tile_variant: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  return kernel == 0 ? lts->tileLocal[cluster] : lts->tileNeighbor[cluster];
 }
 
 extern "C" int sdg_lts_advance(sdg_lts* lts, int macro_steps, void* stream) {

@@ -161,6 +161,26 @@ class LocalTimeStepping:
         fn = lib().sdg_lts_graph_advance if graph else lib().sdg_lts_advance
         check(fn(self.handle, int(macro_steps), s), "sdg_lts_advance")
 
+    def autotune(self, repeats: int = 3, stream=None) -> dict:
+        """Pick the fastest tile variant per cluster and kernel; returns the choices."""
+        import torch
+        s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        check(lib().sdg_lts_autotune(self.handle, int(repeats), s), "sdg_lts_autotune")
+        return self.tiles()
+
+    def set_tile(self, variant: int, cluster: int | None = None, kernel: int | None = None):
+        clusters = range(self.layer.lts.num_clusters) if cluster is None else [cluster]
+        kernels = (0, 1) if kernel is None else (kernel,)
+        for c in clusters:
+            for k in kernels:
+                check(lib().sdg_lts_set_tile_variant(self.handle, c, k, int(variant)), "sdg_lts_set_tile_variant")
+
+    def tiles(self) -> dict:
+        out = {}
+        for c in range(self.layer.lts.num_clusters):
+            out[c] = {k: int(lib().sdg_lts_tile_variant(self.handle, c, i)) for i, k in enumerate(("local", "neigh"))}
+        return out
+
     def close(self):
         if self.handle:
             check(lib().sdg_lts_destroy(self.handle), "sdg_lts_destroy")

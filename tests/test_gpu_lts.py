@@ -50,6 +50,38 @@ def test_lts_matches_oracle(gpu, order, fractions, arith, tol):
     assert err < tol
 
 
+@pytest.mark.parametrize("arith", ["fp64", "fp32"])
+def test_every_tile_variant_and_the_autotuner_keep_the_results(gpu, arith):
+    import torch
+    order = 5
+    lts = make_lts_setup(order, fractions=(0.5, 0.3, 0.2), dims=(10, 6, 6), seed=6)
+    eng = Engine(order, arith)
+    tol = 1e-11 if arith == "fp64" else 2e-5
+    q_ref = None
+    for variant in range(4):
+        layer = DeviceLtsLayer(eng, lts)
+        run = LocalTimeStepping(eng, layer)
+        run.set_tile(variant)
+        run.advance(2)
+        torch.cuda.synchronize()
+        if q_ref is None:
+            q_ref = _oracle_q(lts, order, 2, layer)
+        err = rel_l2(_np(layer.dofs).astype(np.float64), q_ref)
+        print(f"{arith} tile variant {variant}: rel L2 vs oracle {err:.2e}", flush=True)
+        assert err < tol
+    # the autotuner restores the state it ran its trials on
+    layer = DeviceLtsLayer(eng, lts)
+    run = LocalTimeStepping(eng, layer)
+    before = layer.dofs.clone()
+    tiles = run.autotune(repeats=2)
+    torch.cuda.synchronize()
+    assert bool(torch.equal(before, layer.dofs))
+    assert set(tiles) == {0, 1, 2} and all(0 <= v < 4 for t in tiles.values() for v in t.values())
+    run.advance(2, graph=True)
+    torch.cuda.synchronize()
+    assert rel_l2(_np(layer.dofs).astype(np.float64), q_ref) < tol
+
+
 def test_lts_graph_replay_equals_eager(gpu):
     import torch
     order = 4
