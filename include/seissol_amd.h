@@ -312,6 +312,12 @@ int sdg_gauss_legendre(int n, double t0, double t1, double* points, double* weig
  * codegen/matrices/dr_stroud_matrices_{order}.json): points [n][2],
  * weights [n], resample [n][n], n = (order+1)^2. */
 int sdg_dr_quadrature(int order, double* points, double* weights, double* resample);
+/* Face evaluation matrices of dynamic rupture (V3mTo2n(i, h) and
+ * V3mTo2nTWDivM(i, h), codegen/kernels/dynamic_rupture.py:108-190; the data of
+ * codegen/matrices/dr_stroud_matrices_{order}.json), host:
+ * v3m [4][4][n][B] (face i, h = 0 plus side, h = orientation + 1 minus side),
+ * v3mtw [4][4][B][n] or NULL; n = (order+1)^2 Stroud points. */
+int sdg_dr_face_matrices(int order, double* v3m, double* v3mtw);
 /* RateAndStateFastVelocityInitializer::computeInitialStateAndFriction
  * (src/DynamicRupture/Initializer/RateAndStateInitializer.cpp:154-180), host. */
 int sdg_rs_initial_state(double traction1, double traction2, double pressure, double rs_a, double rs_sr0,

@@ -134,6 +134,7 @@ SIGNATURES = {
     "sdg_compute_local_integration": (I32, [P, P, D, C.POINTER(Layer), I32, P]),
     "sdg_compute_neighboring_integration": (I32, [P, P, P, C.POINTER(Layer), P]),
     "sdg_lts_create": (I32, [P, P, C.POINTER(Layer), I32, P, D]),
+    "sdg_dr_face_matrices": (I32, [I32, P, P]),
     "sdg_lts_advance": (I32, [P, I32, P]),
     "sdg_lts_graph_advance": (I32, [P, I32, P]),
     "sdg_lts_destroy": (I32, [P]),

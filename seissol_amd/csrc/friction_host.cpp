@@ -158,3 +158,35 @@ int sdg_rs_initial_state(double traction1, double traction2, double pressure, do
 }
 
 }  // extern "C"
+
+// Face evaluation matrices of dynamic rupture (codegen/kernels/dynamic_rupture.py:
+// V3mTo2n(i, h), V3mTo2nTWDivM(i, h); data of dr_stroud_matrices_{order}.json):
+// v3m [4][4][n][B]: the 3D basis at the n Stroud points of face i, seen with
+// orientation h - 1 (h = 0: the face's own parameterisation, the plus side);
+// v3mtw [4][4][B][n] = v3m^T diag(w) / M (the doubled surface area of the
+// physical face enters through the flux solver).
+extern "C" int sdg_dr_face_matrices(int order, double* v3m, double* v3mtw) {
+  if (order < 1 || order > 8 || v3m == nullptr) {
+    setError("sdg_dr_face_matrices: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  const int n = (order + 1) * (order + 1), b = numBasis3(order);
+  std::vector<double> pts(2 * n), w(n);
+  if (sdg_dr_quadrature(order, pts.data(), w.data(), nullptr) != SDG_OK) return SDG_ERR_INVALID;
+  std::vector<double> gm(globalMatrixLayout(order).total);
+  generateGlobalMatrices(order, gm.data());
+  const double* mass = gm.data() + globalMatrixLayout(order).mass;
+  for (int i = 0; i < 4; ++i)
+    for (int h = 0; h < 4; ++h)
+      for (int k = 0; k < n; ++k) {
+        double x[3];
+        faceToReference(i, pts[2 * k], pts[2 * k + 1], h - 1, x);
+        for (int l = 0; l < b; ++l) {
+          const double v = evalTetBasis(order, l, x[0], x[1], x[2]);
+          v3m[((i * 4 + h) * n + k) * b + l] = v;
+          if (v3mtw != nullptr) v3mtw[((i * 4 + h) * b + l) * n + k] = v * w[k] / mass[l];
+        }
+      }
+  return SDG_OK;
+}
+

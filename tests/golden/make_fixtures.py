@@ -10,7 +10,8 @@ reference at run time.  Only numeric data is extracted:
   materials of tests/Model/Values.h:24-445, 475-477;
 * ``global_matrix_checksums.json`` -- per-matrix sums / norms of
   codegen/matrices/matrices_{4,10,20,35,56,84}.xml;
-* ``dr_stroud.json`` -- quadrature points, weights and the resampling matrix of
+* ``dr_stroud.json`` -- quadrature points, weights, the resampling matrix and
+  the face evaluation matrices V3mTo2n / V3mTo2nTWDivM of
   codegen/matrices/dr_stroud_matrices_{4,5,6}.json (checksums + order-4 data);
 * ``friction_common.json`` -- the known-answer case of
   tests/DynamicRupture/FrictionLaws/FrictionSolverCommon.t.h:20-170: inputs
@@ -147,7 +148,7 @@ def dr_stroud(ref: Path) -> dict:
         data = json.loads((ref / f"codegen/matrices/dr_stroud_matrices_{order}.json").read_text())
         mats = {}
         for m in data:
-            if m["name"] not in ("quadpoints", "quadweights", "resample"):
+            if m["name"] not in ("quadpoints", "quadweights", "resample") and not m["name"].startswith("V3mTo2n"):
                 continue
             a = np.zeros((m["rows"], m["columns"]))
             for r, c, v in m["entries"]:
@@ -155,7 +156,7 @@ def dr_stroud(ref: Path) -> dict:
             mats[m["name"]] = a
         entry = {k: checksums(v) for k, v in mats.items()}
         if order == 4:
-            entry["data"] = {k: v.tolist() for k, v in mats.items()}
+            entry["data"] = {k: v.tolist() for k, v in mats.items() if not k.startswith("V3mTo2n")}
         out[str(order)] = entry
     return out
 

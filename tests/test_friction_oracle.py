@@ -162,3 +162,24 @@ def test_mlp_gradient_matches_finite_differences():
         fd = (oracle.mlp_loss_grad(x, t, p1)[0] - oracle.mlp_loss_grad(x, t, p2)[0]) / (2 * h)
         assert abs(fd - g[idx]) <= 1e-6 * max(1.0, abs(fd)), idx
 
+
+
+def test_dr_face_matrices_match_reference_data():
+    # V3mTo2n(i, h): the basis at the Stroud points of face i, h = 0 plus side,
+    # h = orientation + 1 minus side; V3mTo2nTWDivM(i, h) = V^T diag(w) / M
+    from seissol_amd._capi import check, lib
+    gold = json.loads((GOLDEN / "dr_stroud.json").read_text())
+    for order in (4, 5, 6):
+        n, B = (order + 1) ** 2, order * (order + 1) * (order + 2) // 6
+        v = np.zeros((4, 4, n, B))
+        vt = np.zeros((4, 4, B, n))
+        check(lib().sdg_dr_face_matrices(order, v.ctypes.data, vt.ctypes.data), "sdg_dr_face_matrices")
+        for i in range(4):
+            for h in range(4):
+                for name, a in ((f"V3mTo2n({i},{h})", v[i, h]), (f"V3mTo2nTWDivM({i},{h})", vt[i, h])):
+                    c = gold[str(order)][name]
+                    scale = max(1.0, c["abs_sum"])
+                    assert abs(a.sum() - c["sum"]) < 1e-10 * scale, (order, name)
+                    assert abs(np.abs(a).sum() - c["abs_sum"]) < 1e-10 * scale, (order, name)
+                    assert abs(float((a * np.arange(1, a.size + 1).reshape(a.shape)).sum()) - c["weighted"]) \
+                        < 1e-9 * scale * a.size, (order, name)
