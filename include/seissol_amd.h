@@ -312,6 +312,44 @@ int sdg_gauss_legendre(int n, double t0, double t1, double* points, double* weig
  * codegen/matrices/dr_stroud_matrices_{order}.json): points [n][2],
  * weights [n], resample [n][n], n = (order+1)^2. */
 int sdg_dr_quadrature(int order, double* points, double* weights, double* resample);
+/* A layer of dynamic-rupture faces (DynamicRupture::FaceInformation and
+ * DRGodunovData, src/Initializer/Typedefs.h:213-236).  Element indices refer
+ * to the element layer; 9x9 matrices are column-major FP64. */
+typedef struct {
+  int64_t num_faces;
+  int32_t points;                   /* padded Gauss points per face, >= (order+1)^2 */
+  int32_t pad_;
+  const int32_t* plus_element;      /* [f]                                  */
+  const int32_t* minus_element;     /* [f]                                  */
+  const uint8_t* face_info;         /* [f][3]: plus side, minus side, faceRelation (orientation + 1) */
+  const double* tinv_t;             /* [f][81] TinvT                        */
+  const double* flux_solver_plus;   /* [f][81]                              */
+  const double* flux_solver_minus;  /* [f][81]                              */
+} sdg_dr_layer;
+
+/* Per DR face (DynamicRuptureMatrices.cpp:150-470), host, FP64: TinvT from
+ * the plus side's face frame, the flux solvers fluxScale * A0 * T^T with
+ * fluxScale = -2 area / (6 V_plus) and +2 area / (6 V_minus), and optionally
+ * the impedances imp_and_eta [f][11] of the friction law (layout of
+ * sdg_fault_layer).  vertices [n][4][3], material [n][3] of the cells. */
+int sdg_dr_face_data(int64_t num_faces, const double* vertices, const double* material,
+                     const int32_t* plus_element, const int32_t* minus_element, const uint8_t* face_info,
+                     double* tinv_t, double* flux_solver_plus, double* flux_solver_minus, double* imp_and_eta);
+/* DynamicRupture::batchedSpaceTimeInterpolation (src/Kernels/DynamicRupture.cpp:118-168):
+ * both sides' derivatives (`derivatives` [n][sdg_derivatives_size], the
+ * engine's real type) at the `order` time points (coeffs [order][order]:
+ * c[o][d] = t_o^d / d!) and the Gauss points, rotated to the face frame;
+ * q_plus, q_minus [f][order][9][points] FP64 (the friction law's
+ * q_interpolated_plus/minus).  FP64 and FP32 engines. */
+int sdg_dr_space_time_interpolation(sdg_engine* engine, const double* coeffs, const void* derivatives,
+                                    const sdg_dr_layer* layer, double* q_plus, double* q_minus, void* stream);
+/* Nodal flux of the dynamic-rupture faces (Neighbor.cpp:93-101; spec
+ * dynamic_rupture.py:174-190): for every cell face with dr_mapping [n][4] >= 0,
+ * dofs += V3mTo2nTWDivM(face, h) * imposed state * flux solver of the cell's
+ * side (imposed_plus/minus [f][9][points] FP64, e.g. from sdg_friction_evaluate). */
+int sdg_dr_nodal_flux(sdg_engine* engine, const sdg_layer* layer, const int32_t* dr_mapping,
+                      const sdg_dr_layer* dr_layer, const double* imposed_plus, const double* imposed_minus,
+                      void* stream);
 /* Face evaluation matrices of dynamic rupture (V3mTo2n(i, h) and
  * V3mTo2nTWDivM(i, h), codegen/kernels/dynamic_rupture.py:108-190; the data of
  * codegen/matrices/dr_stroud_matrices_{order}.json), host:

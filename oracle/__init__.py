@@ -40,6 +40,8 @@ def lib() -> C.CDLL:
             "oracle_neighbor": [I, P, I64, P, P, P, P, P, P],
             "oracle_proxy_all": [I, P, I64, I, P, P, P, P, P, P, P, P, P],
             "oracle_lts": [I, P, I64, I, I, P, C.c_double, P, P, P, P, P, P, P, P],
+            "oracle_dr_interpolate": [I, I64, I, P, P, P, P, P, P, P, P, P],
+            "oracle_dr_nodal_flux": [I, I64, I, P, P, P, P, P, P, P, P, P],
         }
         for name, args in sig.items():
             fn = getattr(h, name)
@@ -50,7 +52,10 @@ def lib() -> C.CDLL:
 
 
 def _p(a):
-    return None if a is None else a.ctypes.data
+    """Pointer argument that holds a reference to `a`: a temporary such as
+    ``_p(_c(x))`` (a contiguous copy) then lives until the call returns,
+    where a bare address would point into freed memory."""
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 def _c(a, dt=np.float64):
@@ -107,6 +112,26 @@ def neighbor(order, gm_flat, I, neighbors, face_relations, aminus, dofs, face_ty
     ft = None if face_types is None else _c(face_types, np.uint8)
     lib().oracle_neighbor(order, _p(gm_flat), q.shape[0], _p(_c(I)), _p(_c(neighbors, np.int32)),
                           _p(_c(face_relations, np.uint8)), _p(_c(aminus)), _p(ft), _p(q))
+    return q
+
+
+def dr_interpolate(order, points, v3m, derivatives, plus, minus, info, tinv_t, coeffs):
+    """q_plus, q_minus [f][order][9][points] of the DR faces."""
+    nf = len(plus)
+    qp = np.zeros((nf, order, 9, points))
+    qm = np.zeros_like(qp)
+    lib().oracle_dr_interpolate(order, nf, points, _p(_c(v3m)), _p(_c(derivatives)), _p(_c(plus, np.int32)),
+                                _p(_c(minus, np.int32)), _p(_c(info, np.uint8)), _p(_c(tinv_t)), _p(_c(coeffs)),
+                                _p(qp), _p(qm))
+    return qp, qm
+
+
+def dr_nodal_flux(order, points, v3mtw, dr_mapping, plus, info, flux_plus, flux_minus, imposed_plus, imposed_minus,
+                  dofs):
+    q = _c(dofs).copy()
+    lib().oracle_dr_nodal_flux(order, q.shape[0], points, _p(_c(v3mtw)), _p(_c(dr_mapping, np.int32)),
+                               _p(_c(plus, np.int32)), _p(_c(info, np.uint8)), _p(_c(flux_plus)),
+                               _p(_c(flux_minus)), _p(_c(imposed_plus)), _p(_c(imposed_minus)), _p(q))
     return q
 
 

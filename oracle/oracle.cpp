@@ -479,6 +479,80 @@ int oracle_proxy_all(int order, const double* gm, std::int64_t n, int steps, con
   return 0;
 }
 
+// Dynamic rupture: space-time interpolation of both sides of every DR face
+// (DynamicRupture::spaceTimeInterpolation, src/Kernels/DynamicRupture.cpp:57-109;
+// spec dynamic_rupture.py:108-170): for time point o the Taylor sum of the
+// side's derivatives, evaluated at the face points with V3mTo2n(side, h)
+// (h = 0 plus side, faceRelation minus side) and rotated with TinvT.
+// v3m [4][4][np][B]; coeffs [O][O]; out [f][O][9][points].
+int oracle_dr_interpolate(int order, std::int64_t nf, int points, const double* v3m, const double* derivatives,
+                          const std::int32_t* plus, const std::int32_t* minus, const std::uint8_t* info,
+                          const double* tinvT, const double* coeffs, double* qPlus, double* qMinus) {
+  const int B = nb3(order), np = (order + 1) * (order + 1);
+  const std::int64_t dsize = oracle_derivatives_size(order);
+#pragma omp parallel for schedule(static)
+  for (std::int64_t f = 0; f < nf; ++f) {
+    std::vector<double> Qt(9 * B);
+    for (int side = 0; side < 2; ++side) {
+      const std::int64_t e = side == 0 ? plus[f] : minus[f];
+      const int fs = info[3 * f + side], h = side == 0 ? 0 : info[3 * f + 2];
+      const double* V = v3m + ((fs * 4 + h) * np) * static_cast<std::int64_t>(B);
+      double* out = side == 0 ? qPlus : qMinus;
+      for (int o = 0; o < order; ++o) {
+        oracle_taylor(order, 1, coeffs + o * order, derivatives + e * dsize, Qt.data());
+        for (int k = 0; k < points; ++k)
+          for (int p = 0; p < 9; ++p) {
+            double acc = 0.0;
+            if (k < np)
+              for (int q = 0; q < 9; ++q) {
+                double at = 0.0;
+                for (int l = 0; l < B; ++l) at += V[k * B + l] * Qt[q * B + l];
+                acc += at * tinvT[81 * f + q + 9 * p];
+              }
+            out[((f * order + o) * 9 + p) * points + k] = acc;
+          }
+      }
+    }
+  }
+  return 0;
+}
+
+// Nodal flux of the DR faces into their cells (Neighbor.cpp:93-101; spec
+// dynamic_rupture.py:174-190): for every cell face mapped to DR face m,
+// Q[p][k] += sum_l V3mTo2nTWDivM(face, h)[k][l] sum_q G[q][l] F(q, p) with the
+// imposed state G and flux solver F of the cell's side.  v3mtw [4][4][B][np].
+int oracle_dr_nodal_flux(int order, std::int64_t n, int points, const double* v3mtw, const std::int32_t* drMap,
+                         const std::int32_t* plus, const std::uint8_t* info, const double* fluxPlus,
+                         const double* fluxMinus, const double* impPlus, const double* impMinus, double* dofs) {
+  const int B = nb3(order), np = (order + 1) * (order + 1);
+#pragma omp parallel for schedule(static)
+  for (std::int64_t e = 0; e < n; ++e) {
+    std::vector<double> gf(np * 9);
+    for (int face = 0; face < 4; ++face) {
+      const std::int64_t m = drMap[4 * e + face];
+      if (m < 0) continue;
+      const bool isPlus = plus[m] == e && info[3 * m] == face;
+      const int h = isPlus ? 0 : info[3 * m + 2];
+      const double* G = (isPlus ? impPlus : impMinus) + m * 9 * points;
+      const double* F = (isPlus ? fluxPlus : fluxMinus) + m * 81;
+      for (int l = 0; l < np; ++l)
+        for (int p = 0; p < 9; ++p) {
+          double acc = 0.0;
+          for (int q = 0; q < 9; ++q) acc += G[q * points + l] * F[q + 9 * p];
+          gf[l * 9 + p] = acc;
+        }
+      const double* W = v3mtw + ((face * 4 + h) * static_cast<std::int64_t>(B)) * np;
+      for (int p = 0; p < 9; ++p)
+        for (int k = 0; k < B; ++k) {
+          double acc = 0.0;
+          for (int l = 0; l < np; ++l) acc += W[k * np + l] * gf[l * 9 + p];
+          dofs[(e * 9 + p) * B + k] += acc;
+        }
+    }
+  }
+  return 0;
+}
+
 // Local time stepping with rate-2 clusters (cluster c advances with
 // dt0 * 2^c), restated from TimeCluster::computeLocalIntegration /
 // computeNeighboringIntegration (src/Solver/TimeStepping/TimeCluster.cpp:358-628),

@@ -74,6 +74,14 @@ class FaultLayer(C.Structure):
                 ("pad_", C.c_int32)] + [(name, P) for name in FAULT_FIELDS]
 
 
+class DrLayer(C.Structure):
+    """``sdg_dr_layer``"""
+
+    _fields_ = [("num_faces", I64), ("points", C.c_int32), ("pad_", C.c_int32), ("plus_element", P),
+                ("minus_element", P), ("face_info", P), ("tinv_t", P), ("flux_solver_plus", P),
+                ("flux_solver_minus", P)]
+
+
 class WorkEstimate(C.Structure):
     """``sdg_work_estimate``"""
 
@@ -135,6 +143,9 @@ SIGNATURES = {
     "sdg_compute_neighboring_integration": (I32, [P, P, P, C.POINTER(Layer), P]),
     "sdg_lts_create": (I32, [P, P, C.POINTER(Layer), I32, P, D]),
     "sdg_dr_face_matrices": (I32, [I32, P, P]),
+    "sdg_dr_face_data": (I32, [I64, P, P, P, P, P, P, P, P, P]),
+    "sdg_dr_space_time_interpolation": (I32, [P, P, P, C.POINTER(DrLayer), P, P, P]),
+    "sdg_dr_nodal_flux": (I32, [P, C.POINTER(Layer), P, C.POINTER(DrLayer), P, P, P]),
     "sdg_lts_advance": (I32, [P, I32, P]),
     "sdg_lts_graph_advance": (I32, [P, I32, P]),
     "sdg_lts_destroy": (I32, [P]),

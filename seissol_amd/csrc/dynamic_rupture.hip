@@ -1,0 +1,377 @@
+// Dynamic rupture faces around the friction solve:
+//  * sdg_dr_face_data: per DR face the transposed inverse face rotation, the
+//    two flux solvers and the impedances of the friction law
+//    (src/Initializer/Model/DynamicRuptureMatrices.cpp:150-470);
+//  * kDrInterpolate: space-time interpolation of both sides' derivatives at
+//    the Gauss-Legendre time points and the face quadrature points, rotated to
+//    the face frame (DynamicRupture::spaceTimeInterpolation,
+//    src/Kernels/DynamicRupture.cpp:57-168; spec dynamic_rupture.py:108-170);
+//  * kDrNodalFlux: the flux of the imposed states into the adjacent cells
+//    (Neighbor.cpp:93-101 with the nodalFlux spec, dynamic_rupture.py:174-190).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "cell_local.h"
+#include "engine.h"
+#include "global_matrices.h"
+
+namespace sdg {
+namespace {
+
+constexpr int kDrThreads = 64;
+
+// time-basis coefficients of the O time points: c[o][d] = t_o^d / d!
+template <int O>
+struct StepCoeffs {
+  double c[O][O];
+};
+
+int stroudPoints(int order) { return (order + 1) * (order + 1); }
+
+bool drLayerOk(const sdg_dr_layer* L, int order, const char* who) {
+  if (L == nullptr || L->num_faces < 0) {
+    setError(std::string(who) + ": invalid DR layer");
+    return false;
+  }
+  if (L->num_faces == 0) return true;
+  if (L->points < stroudPoints(order) || L->points > 64 || L->plus_element == nullptr ||
+      L->minus_element == nullptr || L->face_info == nullptr) {
+    setError(std::string(who) + ": DR layer needs points in [(order+1)^2, 64], elements and face info");
+    return false;
+  }
+  return true;
+}
+
+// face matrices of the engine's order on the device (real type of the engine):
+// V3mTo2n [4][4][n][B], V3mTo2nTWDivM [4][4][B][n]
+template <typename real>
+int uploadDrMatrices(EngineState& s) {
+  const int O = s.order, B = numBasis3(O), n = stroudPoints(O);
+  std::vector<double> v(16LL * n * B), vt(16LL * n * B);
+  if (sdg_dr_face_matrices(O, v.data(), vt.data()) != SDG_OK) return SDG_ERR_INVALID;
+  std::vector<real> both(v.size() + vt.size());
+  for (size_t i = 0; i < v.size(); ++i) both[i] = static_cast<real>(v[i]);
+  for (size_t i = 0; i < vt.size(); ++i) both[v.size() + i] = static_cast<real>(vt[i]);
+  SDG_HIP_CHECK(hipMalloc(&s.drMatrices, both.size() * sizeof(real)));
+  SDG_HIP_CHECK(hipMemcpy(s.drMatrices, both.data(), both.size() * sizeof(real), hipMemcpyHostToDevice));
+  return SDG_OK;
+}
+
+int ensureDrMatrices(EngineState& s) {
+  if (s.drMatrices != nullptr) return SDG_OK;
+  return s.arithmetic == SDG_FP64 ? uploadDrMatrices<double>(s) : uploadDrMatrices<float>(s);
+}
+
+// One workgroup per DR face: both sides, all time points.  out[f][o][p][k]
+// (FP64, the friction law's q_interpolated layout) = sum_q (sum_l V[k][l]
+// Q_o[q][l]) TinvT(q, p), Q_o = sum_d c[o][d] dQ_d of the side's cell.
+template <typename real, int O>
+__global__ __launch_bounds__(kDrThreads) void kDrInterpolate(int64_t nf, const real* __restrict__ V,
+                                                             const real* __restrict__ deriv, int64_t dsize,
+                                                             sdg_dr_layer L, StepCoeffs<O> cf,
+                                                             double* __restrict__ outPlus,
+                                                             double* __restrict__ outMinus) {
+  constexpr int B = numBasis3(O), NP = (O + 1) * (O + 1);
+  __shared__ real sQ[9 * B];
+  __shared__ real sT[81];
+  const int64_t f = blockIdx.x;
+  if (f >= nf) return;
+  const int tid = threadIdx.x;
+  const int P = L.points;
+  for (int i = tid; i < 81; i += kDrThreads) sT[i] = static_cast<real>(L.tinv_t[f * 81 + i]);
+  for (int side = 0; side < 2; ++side) {
+    const int64_t e = side == 0 ? L.plus_element[f] : L.minus_element[f];
+    const int faceSide = L.face_info[f * 3 + side];
+    const int h = side == 0 ? 0 : L.face_info[f * 3 + 2];
+    const real* Vf = V + ((static_cast<int64_t>(min(faceSide, 3)) * 4 + min(h, 3)) * NP) * B;
+    double* out = side == 0 ? outPlus : outMinus;
+    for (int o = 0; o < O; ++o) {
+      __syncthreads();
+      for (int idx = tid; idx < 9 * B; idx += kDrThreads) {
+        const int q = idx / B, l = idx % B;
+        real acc = real(0);
+        int64_t off = e * dsize;
+#pragma unroll
+        for (int d = 0; d < O; ++d) {
+          const int m = numBasis3(O - d);
+          if (l < m) acc += static_cast<real>(cf.c[o][d]) * deriv[off + q * m + l];
+          off += 9 * m;
+        }
+        sQ[idx] = acc;
+      }
+      __syncthreads();
+      const int k = tid;
+      if (k < NP) {
+        real at[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) at[q] = real(0);
+        for (int l = 0; l < B; ++l) {
+          const real v = Vf[k * B + l];
+#pragma unroll
+          for (int q = 0; q < 9; ++q) at[q] += v * sQ[q * B + l];
+        }
+#pragma unroll
+        for (int p = 0; p < 9; ++p) {
+          real acc = real(0);
+#pragma unroll
+          for (int q = 0; q < 9; ++q) acc += at[q] * sT[q + 9 * p];
+          out[((f * O + o) * 9 + p) * P + k] = static_cast<double>(acc);
+        }
+      } else if (k < P) {
+        for (int p = 0; p < 9; ++p) out[((f * O + o) * 9 + p) * P + k] = 0.0;
+      }
+    }
+  }
+}
+
+// One workgroup per cell: for each DR face of the cell,
+// Q[p][k] += sum_l VTW[k][l] sum_q G[q][l] F(q, p) with the imposed state G
+// and flux solver F of the cell's side (plus: h = 0; minus: h = faceRelation).
+template <typename real, int O>
+__global__ __launch_bounds__(kDrThreads) void kDrNodalFlux(int64_t n, const real* __restrict__ VTW,
+                                                           const int32_t* __restrict__ drMap, sdg_dr_layer L,
+                                                           const double* __restrict__ impPlus,
+                                                           const double* __restrict__ impMinus,
+                                                           real* __restrict__ dofs) {
+  constexpr int B = numBasis3(O), NP = (O + 1) * (O + 1);
+  __shared__ real sGF[64 * 9];
+  __shared__ real sF[81];
+  const int64_t e = blockIdx.x;
+  if (e >= n) return;
+  const int tid = threadIdx.x;
+  const int P = L.points;
+  real acc[(B + kDrThreads - 1) / kDrThreads][9];
+  for (int r = 0; r < (B + kDrThreads - 1) / kDrThreads; ++r)
+    for (int p = 0; p < 9; ++p) acc[r][p] = real(0);
+  bool any = false;
+  for (int face = 0; face < 4; ++face) {
+    const int32_t m = drMap[e * 4 + face];
+    if (m < 0 || m >= L.num_faces) continue;
+    any = true;
+    const bool plus = L.plus_element[m] == e && L.face_info[m * 3 + 0] == face;
+    const int h = plus ? 0 : min(static_cast<int>(L.face_info[m * 3 + 2]), 3);
+    const double* G = (plus ? impPlus : impMinus) + static_cast<int64_t>(m) * 9 * P;
+    const double* F = (plus ? L.flux_solver_plus : L.flux_solver_minus) + static_cast<int64_t>(m) * 81;
+    __syncthreads();
+    for (int i = tid; i < 81; i += kDrThreads) sF[i] = static_cast<real>(F[i]);
+    __syncthreads();
+    // GF[l][p] = sum_q G[q][l] F(q, p)
+    if (tid < NP) {
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        real s = real(0);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) s += static_cast<real>(G[q * P + tid]) * sF[q + 9 * p];
+        sGF[tid * 9 + p] = s;
+      }
+    }
+    __syncthreads();
+    const real* W = VTW + ((static_cast<int64_t>(face) * 4 + h) * B) * NP;
+    for (int r = 0; r < (B + kDrThreads - 1) / kDrThreads; ++r) {
+      const int k = tid + r * kDrThreads;
+      if (k < B)
+        for (int l = 0; l < NP; ++l) {
+          const real w = W[k * NP + l];
+#pragma unroll
+          for (int p = 0; p < 9; ++p) acc[r][p] += w * sGF[l * 9 + p];
+        }
+    }
+  }
+  if (!any) return;
+  for (int r = 0; r < (B + kDrThreads - 1) / kDrThreads; ++r) {
+    const int k = tid + r * kDrThreads;
+    if (k < B)
+#pragma unroll
+      for (int p = 0; p < 9; ++p) dofs[(e * 9 + p) * B + k] += acc[r][p];
+  }
+}
+
+template <typename real, int O>
+int interpolateImpl(EngineState& st, const double* coeffs, const void* deriv, const sdg_dr_layer* L, double* qp,
+                    double* qm, hipStream_t stream) {
+  if (L->num_faces == 0) return SDG_OK;
+  StepCoeffs<O> cf{};
+  for (int o = 0; o < O; ++o)
+    for (int d = 0; d < O; ++d) cf.c[o][d] = coeffs[o * O + d];
+  int64_t dsize = 0;
+  for (int i = 0; i < O; ++i) dsize += 9LL * numBasis3(O - i);
+  kDrInterpolate<real, O><<<static_cast<unsigned>(L->num_faces), kDrThreads, 0, stream>>>(
+      L->num_faces, static_cast<const real*>(st.drMatrices), static_cast<const real*>(deriv), dsize, *L, cf, qp, qm);
+  SDG_HIP_CHECK(hipGetLastError());
+  return SDG_OK;
+}
+
+template <typename real, int O>
+int nodalFluxImpl(EngineState& st, const sdg_layer* layer, const int32_t* drMap, const sdg_dr_layer* L,
+                  const double* ip, const double* im, hipStream_t stream) {
+  const int64_t n = layer->num_elements;
+  if (n == 0 || L->num_faces == 0) return SDG_OK;
+  const int64_t v3mSize = 16LL * stroudPoints(O) * numBasis3(O);
+  kDrNodalFlux<real, O><<<static_cast<unsigned>(n), kDrThreads, 0, stream>>>(
+      n, static_cast<const real*>(st.drMatrices) + v3mSize, drMap, *L, ip, im, static_cast<real*>(layer->dofs));
+  SDG_HIP_CHECK(hipGetLastError());
+  return SDG_OK;
+}
+
+}  // namespace
+}  // namespace sdg
+
+using namespace sdg;
+
+extern "C" int sdg_dr_face_data(int64_t num_faces, const double* vertices, const double* material,
+                                const int32_t* plus_element, const int32_t* minus_element, const uint8_t* face_info,
+                                double* tinv_t, double* flux_solver_plus, double* flux_solver_minus,
+                                double* imp_and_eta) {
+  if (num_faces < 0 || (num_faces > 0 && (vertices == nullptr || material == nullptr || plus_element == nullptr ||
+                                          minus_element == nullptr || face_info == nullptr || tinv_t == nullptr ||
+                                          flux_solver_plus == nullptr || flux_solver_minus == nullptr))) {
+    setError("sdg_dr_face_data: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  for (int64_t m = 0; m < num_faces; ++m) {
+    const int64_t ep = plus_element[m], em = minus_element[m];
+    const int sp = face_info[m * 3] & 3;
+    double v[4][3], vm[4][3];
+    for (int q = 0; q < 4; ++q)
+      for (int c = 0; c < 3; ++c) {
+        v[q][c] = vertices[(ep * 4 + q) * 3 + c];
+        vm[q][c] = vertices[(em * 4 + q) * 3 + c];
+      }
+    // fault normal and tangents from the plus side's face (MeshTools::normalAndTangents)
+    const double* p0 = v[faceNode(sp, 0)];
+    const double* p1 = v[faceNode(sp, 1)];
+    const double* p2 = v[faceNode(sp, 2)];
+    double ab[3], ac[3], nrm[3], s[3], t[3];
+    for (int k = 0; k < 3; ++k) {
+      ab[k] = p1[k] - p0[k];
+      ac[k] = p2[k] - p0[k];
+    }
+    cross3(ab, ac, nrm);
+    for (int k = 0; k < 3; ++k) s[k] = ab[k];
+    cross3(nrm, s, t);
+    const double nn = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    const double ns = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    const double nt = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    const double area = 0.5 * nn;
+    for (int k = 0; k < 3; ++k) {
+      nrm[k] /= nn;
+      s[k] /= ns;
+      t[k] /= nt;
+    }
+    double T[81], Ti[81], grad[3][3];
+    faceRotation(nrm, s, t, T, Ti);
+    for (int q = 0; q < 9; ++q)
+      for (int p = 0; p < 9; ++p) tinv_t[m * 81 + q + 9 * p] = Ti[p + 9 * q];
+    const double volPlus = std::fabs(referenceGradients(v, grad)) / 6.0;
+    const double volMinus = std::fabs(referenceGradients(vm, grad)) / 6.0;
+    const ElasticMaterial matP{material[3 * ep], material[3 * ep + 1], material[3 * ep + 2]};
+    const ElasticMaterial matM{material[3 * em], material[3 * em + 1], material[3 * em + 2]};
+    // fluxSolver(q, p) = scale * sum_k A0(q, k) T(p, k)  (rotateFluxMatrix)
+    for (int side = 0; side < 2; ++side) {
+      double A0[81];
+      transposedCoefficientMatrix(side == 0 ? matP : matM, 0, A0);
+      const double scale = side == 0 ? -2.0 * area / (6.0 * volPlus) : 2.0 * area / (6.0 * volMinus);
+      double* F = (side == 0 ? flux_solver_plus : flux_solver_minus) + m * 81;
+      for (int q = 0; q < 9; ++q)
+        for (int p = 0; p < 9; ++p) {
+          double acc = 0.0;
+          for (int k = 0; k < 9; ++k) acc += A0[q + 9 * k] * T[p + 9 * k];
+          F[q + 9 * p] = scale * acc;
+        }
+    }
+    if (imp_and_eta != nullptr) {
+      auto speeds = [](const ElasticMaterial& mt, double& vp, double& vs) {
+        vp = std::sqrt((mt.lambda + 2.0 * mt.mu) / mt.rho);
+        vs = std::sqrt(mt.mu / mt.rho);
+      };
+      double vpP, vsP, vpM, vsM;
+      speeds(matP, vpP, vsP);
+      speeds(matM, vpM, vsM);
+      const double zp = matP.rho * vpP, zs = matP.rho * vsP, zpN = matM.rho * vpM, zsN = matM.rho * vsM;
+      double* ie = imp_and_eta + m * 11;
+      ie[0] = zp;
+      ie[1] = zs;
+      ie[2] = zpN;
+      ie[3] = zsN;
+      ie[4] = 1.0 / (1.0 / zp + 1.0 / zpN);
+      ie[5] = 1.0 / (1.0 / zs + 1.0 / zsN);
+      ie[6] = 1.0 / zs + 1.0 / zsN;
+      ie[7] = 1.0 / zp;
+      ie[8] = 1.0 / zs;
+      ie[9] = 1.0 / zpN;
+      ie[10] = 1.0 / zsN;
+    }
+  }
+  return SDG_OK;
+}
+
+extern "C" int sdg_dr_space_time_interpolation(sdg_engine* engine, const double* coeffs, const void* derivatives,
+                                               const sdg_dr_layer* layer, double* q_plus, double* q_minus,
+                                               void* stream) {
+  if (engine == nullptr || coeffs == nullptr) {
+    setError("sdg_dr_space_time_interpolation: NULL argument");
+    return SDG_ERR_INVALID;
+  }
+  auto& st = engine->s;
+  if (!drLayerOk(layer, st.order, "sdg_dr_space_time_interpolation")) return SDG_ERR_INVALID;
+  if (layer->num_faces > 0 && (derivatives == nullptr || q_plus == nullptr || q_minus == nullptr ||
+                               layer->tinv_t == nullptr)) {
+    setError("sdg_dr_space_time_interpolation: derivatives, outputs and TinvT are required");
+    return SDG_ERR_INVALID;
+  }
+  if (st.arithmetic != SDG_FP64 && st.arithmetic != SDG_FP32) {
+    setError("sdg_dr_space_time_interpolation: FP64 and FP32 engines");
+    return SDG_ERR_UNSUPPORTED;
+  }
+  if (const int rc = ensureDrMatrices(st); rc != SDG_OK) return rc;
+  const auto s = static_cast<hipStream_t>(stream);
+  const bool dbl = st.arithmetic == SDG_FP64;
+  switch (st.order) {
+#define SDG_DR_CASE(o)                                                                              \
+  case o:                                                                                           \
+    return dbl ? interpolateImpl<double, o>(st, coeffs, derivatives, layer, q_plus, q_minus, s)   \
+               : interpolateImpl<float, o>(st, coeffs, derivatives, layer, q_plus, q_minus, s);
+    SDG_DR_CASE(2) SDG_DR_CASE(3) SDG_DR_CASE(4) SDG_DR_CASE(5) SDG_DR_CASE(6) SDG_DR_CASE(7)
+#undef SDG_DR_CASE
+    default:
+      setError("sdg_dr_space_time_interpolation: unsupported order");
+      return SDG_ERR_UNSUPPORTED;
+  }
+}
+
+extern "C" int sdg_dr_nodal_flux(sdg_engine* engine, const sdg_layer* layer, const int32_t* dr_mapping,
+                                 const sdg_dr_layer* dr_layer, const double* imposed_plus,
+                                 const double* imposed_minus, void* stream) {
+  if (engine == nullptr || layer == nullptr || layer->num_elements < 0) {
+    setError("sdg_dr_nodal_flux: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  auto& st = engine->s;
+  if (!drLayerOk(dr_layer, st.order, "sdg_dr_nodal_flux")) return SDG_ERR_INVALID;
+  if (layer->num_elements > 0 && dr_layer->num_faces > 0 &&
+      (layer->dofs == nullptr || dr_mapping == nullptr || imposed_plus == nullptr || imposed_minus == nullptr ||
+       dr_layer->flux_solver_plus == nullptr || dr_layer->flux_solver_minus == nullptr)) {
+    setError("sdg_dr_nodal_flux: dofs, dr_mapping, imposed states and flux solvers are required");
+    return SDG_ERR_INVALID;
+  }
+  if (st.arithmetic != SDG_FP64 && st.arithmetic != SDG_FP32) {
+    setError("sdg_dr_nodal_flux: FP64 and FP32 engines");
+    return SDG_ERR_UNSUPPORTED;
+  }
+  if (const int rc = ensureDrMatrices(st); rc != SDG_OK) return rc;
+  const auto s = static_cast<hipStream_t>(stream);
+  const bool dbl = st.arithmetic == SDG_FP64;
+  switch (st.order) {
+#define SDG_DR_CASE(o)                                                                                            \
+  case o:                                                                                                         \
+    return dbl ? nodalFluxImpl<double, o>(st, layer, dr_mapping, dr_layer, imposed_plus, imposed_minus, s)      \
+               : nodalFluxImpl<float, o>(st, layer, dr_mapping, dr_layer, imposed_plus, imposed_minus, s);
+    SDG_DR_CASE(2) SDG_DR_CASE(3) SDG_DR_CASE(4) SDG_DR_CASE(5) SDG_DR_CASE(6) SDG_DR_CASE(7)
+#undef SDG_DR_CASE
+    default:
+      setError("sdg_dr_nodal_flux: unsupported order");
+      return SDG_ERR_UNSUPPORTED;
+  }
+}

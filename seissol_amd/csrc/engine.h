@@ -46,6 +46,8 @@ struct EngineState {
   // matrix-core arithmetics: pre-split 16-bit fragments and FP32 tables
   void* mfmaFrags = nullptr;
   void* mfmaTables = nullptr;
+  // dynamic rupture face matrices (V3mTo2n, V3mTo2nTWDivM), on first use
+  void* drMatrices = nullptr;
 };
 
 }  // namespace sdg

@@ -377,6 +377,7 @@ int sdg_engine_destroy(sdg_engine* engine) {
   if (engine->s.matrices) SDG_HIP_CHECK(hipFree(engine->s.matrices));
   if (engine->s.mfmaFrags) SDG_HIP_CHECK(hipFree(engine->s.mfmaFrags));
   if (engine->s.mfmaTables) SDG_HIP_CHECK(hipFree(engine->s.mfmaTables));
+  if (engine->s.drMatrices) SDG_HIP_CHECK(hipFree(engine->s.drMatrices));
   delete engine;
   return SDG_OK;
 }

@@ -188,8 +188,9 @@ def test_empty_layer_and_errors(gpu):
     import ctypes
     eng = Engine(4, "fp64")
     empty = _capi.Layer(0, None, None, None, None, None, None, None, None, None)
-    _capi.check(_capi.lib().sdg_compute_batched_ader_local(eng.handle, monomial_integrate(4, 0, 1).ctypes.data,
-                                                            1.0, ctypes.byref(empty), None))
+    coeffs = monomial_integrate(4, 0, 1)
+    _capi.check(_capi.lib().sdg_compute_batched_ader_local(eng.handle, coeffs.ctypes.data, 1.0, ctypes.byref(empty),
+                                                            None))
     _capi.check(_capi.lib().sdg_compute_batched_neighbors_integral(eng.handle, ctypes.byref(empty), None))
     bad = _capi.Layer(10, None, None, None, None, None, None, None, None, None)
     assert _capi.lib().sdg_compute_batched_neighbors_integral(eng.handle, ctypes.byref(bad), None) != 0
