@@ -256,8 +256,8 @@ def main(argv=None) -> int:
         print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the element update would shard)", file=sys.stderr)
+        print("bench.py measures one GPU only: the element update has no multi-GPU path here",
+              file=sys.stderr)
         return 2
 
     import torch
