@@ -194,8 +194,10 @@ int sdg_lts_set_tile_variant(sdg_lts* lts, int cluster, int kernel, int variant)
 int sdg_lts_destroy(sdg_lts* lts);
 
 /* Proxy kernel selection (src/Proxy/Common.cpp:22-41, KernelDevice.cpp:144-165):
- * "all", "local", "neigh", "ader", "localwoader".  Runs `steps` repetitions
- * of that kernel on the layer with time-step width dt. */
+ * "all" 0, "local" 1, "neigh" 2, "ader" 3, "localwoader" 4, "neigh_dr" 5,
+ * "godunov_dr" 6, "all_dr" 7 (the enum of src/Proxy/Common.h:18).
+ * sdg_proxy_run runs `steps` repetitions of kernels 0-4 on the layer with
+ * time-step width dt; the DR kernels run through sdg_proxy_run_dr. */
 int sdg_proxy_kernel_id(const char* name);
 int sdg_proxy_run(sdg_engine* engine, int kernel_id, int steps, double dt, const sdg_layer* layer, void* stream);
 
@@ -203,7 +205,10 @@ int sdg_proxy_run(sdg_engine* engine, int kernel_id, int steps, double dt, const
  * src/Monitoring/Metric.h:16-50, summed by src/Proxy/Runner.cpp:120-150),
  * per element: non-zero and hardware FLOP, bytes (DOF blocks and
  * per-element operators moved once) and kernel bytes (what the kernels of
- * this arithmetic move, face traces included). */
+ * this arithmetic move, face traces included).  The DR kernels 5-7 (FP64 and
+ * FP32) count the proxy's layout of 4 DR faces per cell, all faces DR
+ * (DynamicRupture::metrics, src/Kernels/DynamicRupture.cpp:170-195, and the
+ * nodalFlux estimate of Neighbor.cpp:198-214). */
 typedef struct {
   double nonzero_flop;
   double hardware_flop;
@@ -350,6 +355,29 @@ int sdg_dr_space_time_interpolation(sdg_engine* engine, const double* coeffs, co
 int sdg_dr_nodal_flux(sdg_engine* engine, const sdg_layer* layer, const int32_t* dr_mapping,
                       const sdg_dr_layer* dr_layer, const double* imposed_plus, const double* imposed_minus,
                       void* stream);
+/* Dynamic-rupture data of the proxy's DR kernels (the DR storage of
+ * src/Proxy/Allocator.cpp:186-294). */
+typedef struct {
+  const int32_t* dr_mapping;     /* [num_elements][4] DR face of each cell face, -1 none */
+  sdg_dr_layer faces;
+  const double* imposed_plus;    /* [faces][9][points]                     */
+  const double* imposed_minus;   /* [faces][9][points]                     */
+  double* q_plus;                /* [faces][order][9][points] outputs      */
+  double* q_minus;
+} sdg_proxy_dr;
+/* The proxy's random DR faces (Allocator.cpp:257-294), host: 4 n faces
+ * between random cells (plus_element, minus_element [4n]) with random sides
+ * and orientations (face_info [4n][3]), every cell face mapped to a random
+ * DR face (dr_mapping [n][4]).  Deterministic in `seed`. */
+int sdg_proxy_dr_faces(int64_t n, uint64_t seed, int32_t* dr_mapping, int32_t* plus_element,
+                       int32_t* minus_element, uint8_t* face_info);
+/* Proxy kernels 0-7 (KernelDevice.cpp:109-123, KernelHost.cpp:219-244):
+ * neigh_dr = neighbour integral + nodal flux of the DR faces; godunov_dr =
+ * space-time interpolation at the Gauss-Legendre points of [0, dt] from
+ * layer->derivatives; all_dr = ader + local integral (writing the
+ * derivatives), godunov_dr, neigh_dr.  FP64 and FP32 engines for 5-7. */
+int sdg_proxy_run_dr(sdg_engine* engine, int kernel_id, int steps, double dt, const sdg_layer* layer,
+                     const sdg_proxy_dr* dr, void* stream);
 /* Face evaluation matrices of dynamic rupture (V3mTo2n(i, h) and
  * V3mTo2nTWDivM(i, h), codegen/kernels/dynamic_rupture.py:108-190; the data of
  * codegen/matrices/dr_stroud_matrices_{order}.json), host:

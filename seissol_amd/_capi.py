@@ -82,6 +82,13 @@ class DrLayer(C.Structure):
                 ("flux_solver_minus", P)]
 
 
+class ProxyDr(C.Structure):
+    """``sdg_proxy_dr``"""
+
+    _fields_ = [("dr_mapping", P), ("faces", DrLayer), ("imposed_plus", P), ("imposed_minus", P), ("q_plus", P),
+                ("q_minus", P)]
+
+
 class WorkEstimate(C.Structure):
     """``sdg_work_estimate``"""
 
@@ -124,6 +131,8 @@ SIGNATURES = {
     "sdg_compute_batched_ader_local": (I32, [P, P, D, C.POINTER(Layer), P]),
     "sdg_proxy_kernel_id": (I32, [C.c_char_p]),
     "sdg_proxy_run": (I32, [P, I32, I32, D, C.POINTER(Layer), P]),
+    "sdg_proxy_dr_faces": (I32, [I64, C.c_uint64, P, P, P, P]),
+    "sdg_proxy_run_dr": (I32, [P, I32, I32, D, C.POINTER(Layer), C.POINTER(ProxyDr), P]),
     "sdg_upload_reals": (I32, [P, P, P, I64, P]),
     "sdg_download_reals": (I32, [P, P, P, I64, P]),
     "sdg_convert_fp64_to_real": (I32, [P, P, P, I64, P]),

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <random>
 #include <vector>
 
 #include "cell_local.h"
@@ -374,4 +375,73 @@ extern "C" int sdg_dr_nodal_flux(sdg_engine* engine, const sdg_layer* layer, con
       setError("sdg_dr_nodal_flux: unsupported order");
       return SDG_ERR_UNSUPPORTED;
   }
+}
+
+// The proxy's dynamic-rupture data (src/Proxy/Allocator.cpp:186-294): 4 n DR
+// faces between random cells with random sides and orientations, and every
+// cell face mapped to a random DR face.
+extern "C" int sdg_proxy_dr_faces(int64_t n, uint64_t seed, int32_t* dr_mapping, int32_t* plus_element,
+                                  int32_t* minus_element, uint8_t* face_info) {
+  if (n <= 0 || 4 * n > INT32_MAX || dr_mapping == nullptr || plus_element == nullptr || minus_element == nullptr ||
+      face_info == nullptr) {
+    setError("sdg_proxy_dr_faces: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  const int64_t nf = 4 * n;
+  std::mt19937_64 rng(seed);
+  std::uniform_int_distribution<int> side(0, 3), orientation(0, 2);
+  std::uniform_int_distribution<int64_t> cell(0, n - 1), face(0, nf - 1);
+  for (int64_t i = 0; i < 4 * n; ++i) dr_mapping[i] = static_cast<int32_t>(face(rng));
+  for (int64_t f = 0; f < nf; ++f) {
+    plus_element[f] = static_cast<int32_t>(cell(rng));
+    minus_element[f] = static_cast<int32_t>(cell(rng));
+    face_info[3 * f] = static_cast<uint8_t>(side(rng));
+    face_info[3 * f + 1] = static_cast<uint8_t>(side(rng));
+    face_info[3 * f + 2] = static_cast<uint8_t>(orientation(rng) + 1);
+  }
+  return SDG_OK;
+}
+
+// Proxy kernels with dynamic rupture (src/Proxy/KernelDevice.cpp:109-123,
+// KernelHost.cpp:219-244): neigh_dr = neighbour integral of the regular faces
+// + nodal flux of the DR faces; godunov_dr = space-time interpolation at the
+// Gauss-Legendre points of [0, dt]; all_dr = ader + local integral, then
+// godunov_dr, then neigh_dr.
+extern "C" int sdg_proxy_run_dr(sdg_engine* engine, int kernel_id, int steps, double dt, const sdg_layer* layer,
+                                const sdg_proxy_dr* dr, void* stream) {
+  if (engine == nullptr || steps < 0 || kernel_id < 0 || kernel_id > 7) {
+    setError("sdg_proxy_run_dr: invalid arguments");
+    return SDG_ERR_INVALID;
+  }
+  if (kernel_id < 5) return sdg_proxy_run(engine, kernel_id, steps, dt, layer, stream);
+  if (dr == nullptr || layer == nullptr) {
+    setError("sdg_proxy_run_dr: DR kernels need the layer and its DR faces");
+    return SDG_ERR_INVALID;
+  }
+  const int O = engine->s.order;
+  std::vector<double> tp(O), tw(O), coll(O * O), integ(O);
+  if (int rc = sdg_gauss_legendre(O, 0.0, dt, tp.data(), tw.data()); rc != SDG_OK) return rc;
+  for (int o = 0; o < O; ++o) {
+    double fact = 1.0;
+    for (int d = 0; d < O; ++d) {
+      if (d > 0) fact *= d;
+      coll[o * O + d] = std::pow(tp[o], d) / fact;
+    }
+  }
+  sdg_monomial_integrate(O, 0.0, dt, integ.data());
+  for (int s = 0; s < steps; ++s) {
+    int rc = SDG_OK;
+    if (kernel_id == 7) rc = sdg_compute_batched_ader_local(engine, integ.data(), dt, layer, stream);
+    if (rc == SDG_OK && (kernel_id == 6 || kernel_id == 7))
+      rc = sdg_dr_space_time_interpolation(engine, coll.data(), layer->derivatives, &dr->faces, dr->q_plus,
+                                           dr->q_minus, stream);
+    if (rc == SDG_OK && (kernel_id == 5 || kernel_id == 7)) {
+      rc = sdg_compute_batched_neighbors_integral(engine, layer, stream);
+      if (rc == SDG_OK)
+        rc = sdg_dr_nodal_flux(engine, layer, dr->dr_mapping, &dr->faces, dr->imposed_plus, dr->imposed_minus,
+                               stream);
+    }
+    if (rc != SDG_OK) return rc;
+  }
+  return SDG_OK;
 }

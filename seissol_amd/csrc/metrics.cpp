@@ -78,6 +78,37 @@ Work elementWork(int order, const std::vector<double>& gm) {
   return w;
 }
 
+// Dynamic rupture, per DR face: the space-time interpolation of both sides
+// at the `order` time points (Taylor sum, V3mTo2n(i, h) Q, rotation by
+// TinvT) and the nodal flux of one side (G F, V3mTo2nTWDivM(i, h) (G F));
+// V3mTo2n counts averaged over the plus (h = 0) and minus (h = 1..3) forms.
+struct DrWork {
+  Counts godunovFace, nodalFace;
+};
+
+DrWork drWork(int order) {
+  const int B = numBasis3(order), np = (order + 1) * (order + 1);
+  std::vector<double> v(16LL * np * B), vt(16LL * np * B);
+  sdg_dr_face_matrices(order, v.data(), vt.data());
+  double nzPlus = 0, nzMinus = 0, nzTwPlus = 0, nzTwMinus = 0;
+  for (int i = 0; i < 4; ++i)
+    for (int h = 0; h < 4; ++h) {
+      const double a = nonZeros(&v[(i * 4 + h) * static_cast<size_t>(np) * B], np, B, B);
+      const double b = nonZeros(&vt[(i * 4 + h) * static_cast<size_t>(np) * B], B, np, np);
+      (h == 0 ? nzPlus : nzMinus) += h == 0 ? a / 4.0 : a / 12.0;
+      (h == 0 ? nzTwPlus : nzTwMinus) += h == 0 ? b / 4.0 : b / 12.0;
+    }
+  DrWork w;
+  double taylor = 0;
+  for (int d = 0; d < order; ++d) taylor += 2.0 * 9 * numBasis3(order - d);
+  const double rot = 2.0 * np * 81;
+  w.godunovFace.nz = order * (2 * taylor + 2.0 * (nzPlus + nzMinus) * 9 + 2 * rot);
+  w.godunovFace.dense = order * (2 * taylor + 2.0 * 2 * np * B * 9 + 2 * rot);
+  w.nodalFace.nz = rot + 2.0 * 0.5 * (nzTwPlus + nzTwMinus) * 9;
+  w.nodalFace.dense = rot + 2.0 * B * np * 9;
+  return w;
+}
+
 }  // namespace
 }  // namespace sdg
 
@@ -85,7 +116,8 @@ using namespace sdg;
 
 extern "C" int sdg_performance_estimate(int order, int arithmetic, int kernel_id, sdg_work_estimate* out) {
   if (out == nullptr || order < kMinOrder || order > kMaxOrder || arithmetic < SDG_FP64 ||
-      arithmetic > SDG_F16X3 || kernel_id < 0 || kernel_id > 4) {
+      arithmetic > SDG_F16X3 || kernel_id < 0 || kernel_id > 7 ||
+      (kernel_id >= 5 && arithmetic != SDG_FP64 && arithmetic != SDG_FP32)) {
     setError("sdg_performance_estimate: invalid order, arithmetic, kernel or NULL output");
     return SDG_ERR_INVALID;
   }
@@ -106,6 +138,35 @@ extern "C" int sdg_performance_estimate(int order, int arithmetic, int kernel_id
     c.nz += x.nz;
     c.dense += x.dense;
   };
+  if (kernel_id >= 5) {
+    // the proxy's DR layer: all four faces of every cell are DR faces and
+    // there are 4 DR faces per cell (src/Proxy/Allocator.cpp:193, :230)
+    const DrWork d = drWork(order);
+    const int np = (order + 1) * (order + 1);
+    const double dsize = static_cast<double>(sdg_derivatives_size(order));
+    const double godunovBytes = 4 * (2 * dsize * s + 81 * 8 + 2.0 * order * 9 * np * 8);
+    const double nodalBytes = 2 * q * s + 4 * (9.0 * np * 8 + 81 * 8);
+    if (kernel_id == 7) {
+      add(w.ader);
+      add(w.volume);
+      bytes += (3 * q + 3 * kStarNonZeros + 4 * 81 + dsize) * s;  // Q in/out, I, derivatives out
+    }
+    if (kernel_id == 6 || kernel_id == 7) {
+      c.nz += 4 * d.godunovFace.nz;
+      c.dense += 4 * d.godunovFace.dense;
+      bytes += godunovBytes;
+    }
+    if (kernel_id == 5 || kernel_id == 7) {
+      c.nz += 4 * d.nodalFace.nz;
+      c.dense += 4 * d.nodalFace.dense;
+      bytes += nodalBytes;
+    }
+    out->nonzero_flop = c.nz;
+    out->hardware_flop = c.dense;
+    out->bytes = bytes;
+    out->kernel_bytes = bytes;
+    return SDG_OK;
+  }
   const bool ader = kernel_id == 0 || kernel_id == 1 || kernel_id == 3;
   const bool local = kernel_id == 0 || kernel_id == 1 || kernel_id == 4;
   const bool neigh = kernel_id == 0 || kernel_id == 2;

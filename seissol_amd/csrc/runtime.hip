@@ -430,13 +430,17 @@ int sdg_compute_batched_neighbors_integral(sdg_engine* engine, const sdg_layer* 
 
 int sdg_proxy_kernel_id(const char* name) {
   if (name == nullptr) return -1;
-  const char* names[] = {"all", "local", "neigh", "ader", "localwoader"};
-  for (int i = 0; i < 5; ++i)
+  const char* names[] = {"all", "local", "neigh", "ader", "localwoader", "neigh_dr", "godunov_dr", "all_dr"};
+  for (int i = 0; i < 8; ++i)
     if (std::strcmp(name, names[i]) == 0) return i;
   return -1;
 }
 
 int sdg_proxy_run(sdg_engine* engine, int kernel_id, int steps, double dt, const sdg_layer* layer, void* stream) {
+  if (kernel_id >= 5 && kernel_id <= 7) {
+    setError("sdg_proxy_run: the DR kernels need DR faces; call sdg_proxy_run_dr");
+    return SDG_ERR_INVALID;
+  }
   if (engine == nullptr || steps < 0 || kernel_id < 0 || kernel_id > 4 || !validLayer(layer, kernel_id == 0 || kernel_id == 2)) {
     if (engine == nullptr || steps < 0 || kernel_id < 0 || kernel_id > 4) setError("sdg_proxy_run: invalid arguments");
     return SDG_ERR_INVALID;

@@ -4,8 +4,11 @@
 //   seissol_amd_proxy [--order O] [--arithmetic fp64|fp32|bf16x3|bf16x6|f16x3]
 //                     [--mesh proxy|periodic] [--seed S] [-f plain|json]
 //                     <cells> <timesteps> <kernel[,kernel...]>
-// Kernels: all, local, neigh, ader, localwoader (src/Proxy/Common.cpp:22-41);
-// a comma-separated list runs its kernels in sequence in every pass.
+// Kernels: all, local, neigh, ader, localwoader, neigh_dr, godunov_dr, all_dr
+// (src/Proxy/Common.cpp:22-41); a comma-separated list runs its kernels in
+// sequence in every pass.  With a DR kernel every cell face is a DR face and
+// the layer gets the proxy's random DR faces, derivatives and imposed states
+// (src/Proxy/Allocator.cpp:186-294; FP64 and FP32).
 //
 // Data as src/Proxy/Allocator.cpp:49-134 (random affine tetrahedra with
 // random neighbours and face relations; or a conforming periodic box),
@@ -86,7 +89,7 @@ int usage(const char* msg) {
   if (msg != nullptr) std::cerr << msg << "\n";
   std::cerr << "usage: seissol_amd_proxy [--order O] [--arithmetic fp64|fp32|bf16x3|bf16x6|f16x3] "
                "[--mesh proxy|periodic] [--seed S] [-f plain|json] <cells> <timesteps> <kernel[,kernel...]>\n"
-               "kernels: all, local, neigh, ader, localwoader\n";
+               "kernels: all, local, neigh, ader, localwoader, neigh_dr, godunov_dr, all_dr\n";
   return -1;
 }
 
@@ -167,6 +170,9 @@ int run(int argc, char** argv) {
     kernelIds.push_back(id);
   }
   const Arithmetic arithmetic = seissol_amd::arithmeticFromString(arith);
+  const bool withDR = std::any_of(kernelIds.begin(), kernelIds.end(), [](int id) { return id >= 5; });
+  if (withDR && arithmetic != Arithmetic::FP64 && arithmetic != Arithmetic::FP32)
+    return usage("the DR kernels run in fp64 or fp32");
 
   // synthetic data (host, FP64)
   std::cerr << "Allocating fake data..." << std::endl;
@@ -179,7 +185,7 @@ int run(int argc, char** argv) {
   const int B = order * (order + 1) * (order + 2) / 6, B2 = order * (order + 1) / 2;
   std::vector<double> vertices(n * 12), material(n * 3), dofs(n * 9 * B);
   std::vector<int32_t> neighbors(n * 4);
-  std::vector<uint8_t> relations(n * 8), faceTypes(n * 4, SDG_FACE_REGULAR);
+  std::vector<uint8_t> relations(n * 8), faceTypes(n * 4, withDR ? SDG_FACE_DYNAMIC_RUPTURE : SDG_FACE_REGULAR);
   if (mesh == "proxy")
     check(sdg_mesh_proxy(cells, seed, vertices.data(), neighbors.data(), relations.data()), "sdg_mesh_proxy");
   else
@@ -220,23 +226,66 @@ int run(int argc, char** argv) {
   const bool traces = arithmetic == Arithmetic::BF16x3 || arithmetic == Arithmetic::BF16x6 ||
                       arithmetic == Arithmetic::F16x3;
   void* dTraces = traces ? deviceZeros(n * 4 * 9 * B2 * 4) : nullptr;
+  // dynamic rupture: random DR faces, derivatives and imposed states
+  void* dDeriv = nullptr;
+  std::vector<void*> drBuffers;
+  sdg_proxy_dr dr{};
+  if (withDR) {
+    const int64_t dsize = sdg_derivatives_size(order), nf = 4 * cells, points = (order + 1) * (order + 1);
+    std::vector<double> deriv(n * dsize);
+    check(sdg_uniform_dofs(static_cast<int64_t>(deriv.size()), seed + 5, deriv.data()), "sdg_uniform_dofs");
+    dDeriv = deviceZeros(deriv.size() * real);
+    check(sdg_upload_reals(global.engine(), dDeriv, deriv.data(), static_cast<int64_t>(deriv.size()), nullptr),
+          "sdg_upload_reals");
+    std::vector<int32_t> mapping(n * 4), plus(nf), minus(nf);
+    std::vector<uint8_t> info(nf * 3);
+    check(sdg_proxy_dr_faces(cells, seed + 3, mapping.data(), plus.data(), minus.data(), info.data()),
+          "sdg_proxy_dr_faces");
+    std::vector<double> tinv(nf * 81), fluxPlus(nf * 81), fluxMinus(nf * 81), imposed(2 * nf * 9 * points);
+    check(sdg_dr_face_data(nf, vertices.data(), material.data(), plus.data(), minus.data(), info.data(), tinv.data(),
+                           fluxPlus.data(), fluxMinus.data(), nullptr),
+          "sdg_dr_face_data");
+    check(sdg_uniform_dofs(static_cast<int64_t>(imposed.size()), seed + 4, imposed.data()), "sdg_uniform_dofs");
+    auto keep = [&](void* p) {
+      drBuffers.push_back(p);
+      return p;
+    };
+    dr.dr_mapping = static_cast<int32_t*>(keep(deviceCopy(mapping)));
+    dr.faces.num_faces = nf;
+    dr.faces.points = static_cast<int32_t>(points);
+    dr.faces.plus_element = static_cast<int32_t*>(keep(deviceCopy(plus)));
+    dr.faces.minus_element = static_cast<int32_t*>(keep(deviceCopy(minus)));
+    dr.faces.face_info = static_cast<uint8_t*>(keep(deviceCopy(info)));
+    dr.faces.tinv_t = static_cast<double*>(keep(deviceCopy(tinv)));
+    dr.faces.flux_solver_plus = static_cast<double*>(keep(deviceCopy(fluxPlus)));
+    dr.faces.flux_solver_minus = static_cast<double*>(keep(deviceCopy(fluxMinus)));
+    double* dImposed = static_cast<double*>(keep(deviceCopy(imposed)));
+    dr.imposed_plus = dImposed;
+    dr.imposed_minus = dImposed + nf * 9 * points;
+    dr.q_plus = static_cast<double*>(keep(deviceZeros(nf * order * 9 * points * 8)));
+    dr.q_minus = static_cast<double*>(keep(deviceZeros(nf * order * 9 * points * 8)));
+  }
   hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
-  sdg_layer layer{cells, dDofs, dInt, nullptr, star, aplus, aminus, dNb, dRel, dTypes, dTraces};
+  sdg_layer layer{cells, dDofs, dInt, dDeriv, star, aplus, aminus, dNb, dRel, dTypes, dTraces, nullptr};
   std::cerr << "...done" << std::endl;
 
   seissol_amd::StreamRuntime runtime{};
   hipStream_t stream;
   hipCheck(hipStreamCreate(&stream), "hipStreamCreate");
   runtime.stream = stream;
+  auto run1 = [&](int id, int steps) {
+    if (withDR)
+      check(sdg_proxy_run_dr(global.engine(), id, steps, dt, &layer, &dr, runtime.stream), "sdg_proxy_run_dr");
+    else
+      check(sdg_proxy_run(global.engine(), id, steps, dt, &layer, runtime.stream), "sdg_proxy_run");
+  };
   auto pass = [&](int64_t steps) {
     if (kernelIds.size() == 1) {
-      check(sdg_proxy_run(global.engine(), kernelIds[0], static_cast<int>(steps), dt, &layer, runtime.stream),
-            "sdg_proxy_run");
+      run1(kernelIds[0], static_cast<int>(steps));
       return;
     }
     for (int64_t t = 0; t < steps; ++t)
-      for (int id : kernelIds)
-        check(sdg_proxy_run(global.engine(), id, 1, dt, &layer, runtime.stream), "sdg_proxy_run");
+      for (int id : kernelIds) run1(id, 1);
   };
   pass(1);  // untimed pass
   hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -265,8 +314,9 @@ int run(int argc, char** argv) {
   hipStreamDestroy(stream);
   for (void* p : {static_cast<void*>(dVert), static_cast<void*>(dMat), static_cast<void*>(dNb),
                   static_cast<void*>(dRel), static_cast<void*>(dTypes), static_cast<void*>(star64),
-                  static_cast<void*>(aplus64), static_cast<void*>(aminus64), dDofs, dInt, dTraces})
+                  static_cast<void*>(aplus64), static_cast<void*>(aminus64), dDofs, dInt, dTraces, dDeriv})
     if (p != nullptr) hipFree(p);
+  for (void* p : drBuffers) hipFree(p);
   if (real == 4) {
     hipFree(star);
     hipFree(aplus);

@@ -6,6 +6,8 @@
 //   seissol_amd::kernels::Time::evaluateBatched           src/Kernels/Time.h:42-61
 //   seissol_amd::kernels::Local::computeBatchedIntegral   src/Kernels/Local.h:58-79
 //   seissol_amd::kernels::Neighbor::computeBatchedNeighborsIntegral  src/Kernels/Neighbor.h:44-55
+//     (with the nodal flux of the layer's dynamic-rupture faces, Neighbor.cpp:93-101)
+//   seissol_amd::kernels::DynamicRupture::batchedSpaceTimeInterpolation  src/Kernels/DynamicRupture.h:30-58
 //   *::metrics()                                 src/Monitoring/Metric.h:16-50
 //   seissol_amd::dr::friction_law::FrictionSolver  src/DynamicRupture/FrictionLaws/FrictionSolver.h:28-108
 //
@@ -145,7 +147,34 @@ class Neighbor {
     check(sdg_compute_batched_neighbors_integral(global_->engine(), &layer, rt.stream),
           "Neighbor::computeBatchedNeighborsIntegral");
   }
+  /// regular faces, then the nodal flux of the dynamic-rupture faces from
+  /// their imposed states (imposedPlus/Minus [faces][9][points], FP64)
+  void computeBatchedNeighborsIntegral(const sdg_layer& layer, const int32_t* drMapping, const sdg_dr_layer& faces,
+                                       const double* imposedPlus, const double* imposedMinus, StreamRuntime& rt) {
+    computeBatchedNeighborsIntegral(layer, rt);
+    check(sdg_dr_nodal_flux(global_->engine(), &layer, drMapping, &faces, imposedPlus, imposedMinus, rt.stream),
+          "Neighbor::computeBatchedNeighborsIntegral (nodal flux)");
+  }
   PerformanceEstimate metrics() const { return estimate(global_->order(), global_->arithmetic(), 2); }
+
+ private:
+  const GlobalData* global_ = nullptr;
+};
+
+/// Space-time interpolation of the dynamic-rupture faces (DynamicRupture.h:30-58).
+class DynamicRupture {
+ public:
+  void setGlobalData(const GlobalData& g) { global_ = &g; }
+  /// coeffsCollocate [order][order] (c[o][d] = t_o^d / d!), derivatives of the
+  /// layer's cells; qPlus/qMinus [faces][order][9][points] FP64
+  void batchedSpaceTimeInterpolation(const sdg_dr_layer& faces, const double* coeffsCollocate,
+                                     const void* derivatives, double* qPlus, double* qMinus, StreamRuntime& rt) {
+    check(sdg_dr_space_time_interpolation(global_->engine(), coeffsCollocate, derivatives, &faces, qPlus, qMinus,
+                                          rt.stream),
+          "DynamicRupture::batchedSpaceTimeInterpolation");
+  }
+  /// work per DR face (the proxy's godunov_dr estimate counts 4 faces per cell)
+  PerformanceEstimate metrics() const { return estimate(global_->order(), global_->arithmetic(), 6) * 0.25; }
 
  private:
   const GlobalData* global_ = nullptr;

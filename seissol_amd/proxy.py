@@ -21,10 +21,11 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _capi
-from ._capi import Layer, check, lib, ptr
+from ._capi import DrLayer, Layer, ProxyDr, check, lib, ptr
 from .kernels import MATRIX_CORE, Engine, derivatives_size, num_basis, num_face_basis
 
-KERNELS = ("all", "local", "neigh", "ader", "localwoader")
+KERNELS = ("all", "local", "neigh", "ader", "localwoader", "neigh_dr", "godunov_dr", "all_dr")
+DR_KERNELS = ("neigh_dr", "godunov_dr", "all_dr")
 
 
 @dataclass
@@ -144,6 +145,60 @@ class DeviceLayer:
         self.dofs.copy_(torch.from_numpy(dofs).to(self.dofs.device).to(self.dofs.dtype))
 
 
+class DeviceDrProxy:
+    """The proxy's dynamic-rupture data on the device (src/Proxy/Allocator.cpp:186-294):
+    every face of every cell is a DR face, 4 n DR faces join random cells with
+    random sides and orientations, the layer's derivatives hold random values
+    (the proxy's fake derivatives) and the imposed states random values.
+    The layer must carry derivatives (``DeviceLayer(..., with_derivatives=True)``)."""
+
+    def __init__(self, engine: Engine, layer: "DeviceLayer", seed: int | None = None, device: str = "cuda"):
+        import torch
+
+        setup = layer.setup
+        if layer.derivatives is None:
+            raise ValueError("the DR proxy kernels need a layer with derivatives")
+        n, order = setup.n, setup.order
+        seed = setup.seed + 3 if seed is None else seed
+        nf = 4 * n
+        mapping = np.zeros((n, 4), np.int32)
+        plus = np.zeros(nf, np.int32)
+        minus = np.zeros(nf, np.int32)
+        info = np.zeros((nf, 3), np.uint8)
+        check(lib().sdg_proxy_dr_faces(n, seed, mapping.ctypes.data, plus.ctypes.data, minus.ctypes.data,
+                                       info.ctypes.data), "sdg_proxy_dr_faces")
+        tinv = np.zeros((nf, 81))
+        fp = np.zeros((nf, 81))
+        fm = np.zeros((nf, 81))
+        v = np.ascontiguousarray(setup.vertices)
+        m = np.ascontiguousarray(setup.material)
+        check(lib().sdg_dr_face_data(nf, v.ctypes.data, m.ctypes.data, plus.ctypes.data, minus.ctypes.data,
+                                     info.ctypes.data, tinv.ctypes.data, fp.ctypes.data, fm.ctypes.data, None),
+              "sdg_dr_face_data")
+        points = (order + 1) ** 2
+        dev = torch.device(device)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        self.mapping, self.plus, self.minus, self.info = t(mapping), t(plus), t(minus), t(info)
+        self.tinv_t, self.flux_plus, self.flux_minus = t(tinv), t(fp), t(fm)
+        rnd = np.zeros(nf * 9 * points * 2)
+        check(lib().sdg_uniform_dofs(rnd.size, seed + 1, rnd.ctypes.data), "sdg_uniform_dofs")
+        self.imposed_plus = t(rnd[: nf * 9 * points].reshape(nf, 9, points))
+        self.imposed_minus = t(rnd[nf * 9 * points:].reshape(nf, 9, points))
+        self.q_plus = torch.zeros((nf, order, 9, points), dtype=torch.float64, device=dev)
+        self.q_minus = torch.zeros_like(self.q_plus)
+        der = np.zeros(layer.derivatives.numel())
+        check(lib().sdg_uniform_dofs(der.size, seed + 2, der.ctypes.data), "sdg_uniform_dofs")
+        layer.derivatives.copy_(torch.from_numpy(der).reshape(layer.derivatives.shape).to(layer.derivatives.dtype))
+        layer.face_types.fill_(_capi.SDG_FACE_DYNAMIC_RUPTURE)
+        self.faces = DrLayer(nf, points, 0, ptr(self.plus), ptr(self.minus), ptr(self.info), ptr(self.tinv_t),
+                             ptr(self.flux_plus), ptr(self.flux_minus))
+        self.view = ProxyDr(ptr(self.mapping), self.faces, ptr(self.imposed_plus), ptr(self.imposed_minus),
+                            ptr(self.q_plus), ptr(self.q_minus))
+        self.num_faces, self.points = nf, points
+        self.host = {"dr_mapping": mapping, "plus": plus, "minus": minus, "face_info": info, "tinv_t": tinv,
+                     "flux_plus": fp, "flux_minus": fm}
+
+
 def kernel_id(name: str) -> int:
     k = lib().sdg_proxy_kernel_id(name.encode())
     if k < 0:
@@ -151,9 +206,17 @@ def kernel_id(name: str) -> int:
     return k
 
 
-def run_proxy(engine: Engine, layer: DeviceLayer, kernel: str, steps: int, dt: float, stream=None):
-    """Enqueue `steps` repetitions of proxy kernel `kernel` (no synchronisation)."""
+def run_proxy(engine: Engine, layer: DeviceLayer, kernel: str, steps: int, dt: float, stream=None,
+              dr: DeviceDrProxy | None = None):
+    """Enqueue `steps` repetitions of proxy kernel `kernel` (no synchronisation);
+    the DR kernels take the layer's DR data `dr`."""
     import torch
     s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+    if kernel in DR_KERNELS:
+        if dr is None:
+            raise ValueError(f"proxy kernel {kernel!r} needs the layer's DR data (DeviceDrProxy)")
+        check(lib().sdg_proxy_run_dr(engine.handle, kernel_id(kernel), steps, float(dt), C.byref(layer.view),
+                                     C.byref(dr.view), s), "sdg_proxy_run_dr")
+        return
     check(lib().sdg_proxy_run(engine.handle, kernel_id(kernel), steps, float(dt), C.byref(layer.view), s),
           "sdg_proxy_run")

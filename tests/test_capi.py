@@ -3,6 +3,8 @@ import ctypes
 import re
 from pathlib import Path
 
+import numpy as np
+
 from seissol_amd import _capi
 
 HEADER = Path(__file__).resolve().parents[1] / "include" / "seissol_amd.h"
@@ -63,7 +65,63 @@ def test_work_estimates_match_the_reference_sparsity():
     assert abs(a.nonzero_flop - sum(p.nonzero_flop for p in parts)) < 1e-6
     # split-bf16 forms execute 3 or 6 products per dense contraction
     assert _estimate(6, _capi.SDG_BF16X6, 0).hardware_flop == 6 * _estimate(6, _capi.SDG_FP32, 0).hardware_flop
-    assert _capi.lib().sdg_performance_estimate(6, 0, 7, ctypes.byref(_capi.WorkEstimate())) != 0
+    assert _capi.lib().sdg_performance_estimate(6, 0, 8, ctypes.byref(_capi.WorkEstimate())) != 0
+
+
+def test_dr_work_estimates_known_answers():
+    # the proxy's DR layer: 4 DR faces per cell, all cell faces DR
+    order, B, NP = 6, 56, 49
+    taylor = 2 * 9 * (56 + 35 + 20 + 10 + 4 + 1)   # Taylor sum of one side at one time point
+    rot = 2 * NP * 81                               # rotation by TinvT / product with the flux solver
+    godunov_dense = 4 * order * (2 * taylor + 2 * 2 * NP * B * 9 + 2 * rot)
+    nodal_dense = 4 * (rot + 2 * B * NP * 9)
+    g = _estimate(order, _capi.SDG_FP64, 6)
+    nb = _estimate(order, _capi.SDG_FP64, 5)
+    assert g.hardware_flop == godunov_dense == 2860704
+    assert nb.hardware_flop == nodal_dense == 229320
+    # non-zero counts from the sparsity of the face matrices (entries below
+    # 1e-13 of the matrix maximum are zero): plus side h = 0, minus side the
+    # mean over h = 1..3, averaged over the four faces
+    from seissol_amd.dr import face_matrices
+    v, vt = face_matrices(order)
+    nz = lambda m: np.count_nonzero(np.abs(m) > 1e-13 * np.abs(m).max())
+    nz_v = np.mean([nz(v[i, 0]) for i in range(4)]) + np.mean([nz(v[i, h]) for i in range(4) for h in (1, 2, 3)])
+    nz_t = 0.5 * (np.mean([nz(vt[i, 0]) for i in range(4)]) +
+                  np.mean([nz(vt[i, h]) for i in range(4) for h in (1, 2, 3)]))
+    assert nz_v < 2 * NP * B
+    assert abs(g.nonzero_flop - 4 * order * (2 * taylor + 2 * nz_v * 9 + 2 * rot)) < 1e-6
+    assert abs(nb.nonzero_flop - 4 * (rot + 2 * nz_t * 9)) < 1e-6
+    dsize = 9 * (56 + 35 + 20 + 10 + 4 + 1)
+    assert g.bytes == 4 * (2 * dsize * 8 + 81 * 8 + 2 * order * 9 * NP * 8)
+    assert nb.bytes == 2 * 9 * B * 8 + 4 * (9 * NP * 8 + 81 * 8)
+    # all_dr = ader + volume (no local flux on DR faces) + godunov_dr + neigh_dr
+    volume_nz = 38808
+    a = _estimate(order, _capi.SDG_FP64, 7)
+    ader = _estimate(order, _capi.SDG_FP64, 3)
+    assert abs(a.nonzero_flop - (ader.nonzero_flop + volume_nz + g.nonzero_flop + nb.nonzero_flop)) < 1e-6
+    # FP32 halves the cell data, not the FP64 face data
+    assert _estimate(order, _capi.SDG_FP32, 5).bytes == 2 * 9 * B * 4 + 4 * (9 * NP * 8 + 81 * 8)
+    # the DR kernels run in FP64 and FP32 only
+    assert _capi.lib().sdg_performance_estimate(6, _capi.SDG_F16X3, 7, ctypes.byref(_capi.WorkEstimate())) != 0
+
+
+def test_proxy_kernel_names_and_dr_faces():
+    lib = _capi.lib()
+    names = ["all", "local", "neigh", "ader", "localwoader", "neigh_dr", "godunov_dr", "all_dr"]
+    assert [lib.sdg_proxy_kernel_id(n.encode()) for n in names] == list(range(8))
+    assert lib.sdg_proxy_kernel_id(b"dr") == -1
+    n = 500
+    out = [np.zeros((n, 4), np.int32), np.zeros(4 * n, np.int32), np.zeros(4 * n, np.int32),
+           np.zeros((4 * n, 3), np.uint8)]
+    _capi.check(lib.sdg_proxy_dr_faces(n, 7, *[a.ctypes.data for a in out]))
+    mapping, plus, minus, info = out
+    assert mapping.min() >= 0 and mapping.max() < 4 * n
+    assert plus.min() >= 0 and plus.max() < n and minus.min() >= 0 and minus.max() < n
+    assert set(np.unique(info[:, :2])) == {0, 1, 2, 3} and set(np.unique(info[:, 2])) == {1, 2, 3}
+    again = [np.zeros_like(a) for a in out]
+    _capi.check(lib.sdg_proxy_dr_faces(n, 7, *[a.ctypes.data for a in again]))
+    assert all(np.array_equal(a, b) for a, b in zip(out, again))
+    assert lib.sdg_proxy_dr_faces(0, 7, *[a.ctypes.data for a in out]) != 0
 
 
 def test_proxy_cli_rejects_bad_arguments():

@@ -103,7 +103,7 @@ def test_welded_fault_through_the_gpu_chain_equals_the_regular_update(gpu, order
     ft0 = np.zeros_like(ft_dr)
     star, ap, am = oracle.cell_local_matrices(setup.vertices, setup.material, setup.neighbors, ft0)
     coeffs = monomial_integrate(order, 0.0, setup.dt)
-    I, _ = oracle.ader(order, gm, coeffs, setup.dofs, star)
+    I = oracle.ader(order, gm, coeffs, setup.dofs, star)
     q_reg = oracle.neighbor(order, gm, I, setup.neighbors, setup.face_relations, am,
                             oracle.local(order, gm, I, star, ap, setup.dofs, ft0), ft0)
     # the GPU chain with DR faces
@@ -151,3 +151,54 @@ def test_dr_entry_points_reject_bad_layers(gpu):
     mc = Engine(3, "f16x3")
     with pytest.raises(_capi.SdgError):
         dr.interpolate(mc, c, layer.derivatives, qp, qp)
+
+
+@pytest.mark.parametrize("arith,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+def test_proxy_dr_kernels_match_oracle(gpu, arith, tol):
+    """godunov_dr, neigh_dr and all_dr on the proxy's random DR layer (every
+    cell face a DR face, 4 n faces between random cells)."""
+    import torch
+    from seissol_amd.proxy import DeviceDrProxy, run_proxy
+    order = 4
+    setup = make_setup(order, n=400, mesh="proxy", seed=5)
+    print(f"proxy dr kernels {arith} n={setup.n}", flush=True)
+    eng = Engine(order, arith)
+    layer = DeviceLayer(eng, setup, with_derivatives=True)
+    dr = DeviceDrProxy(eng, layer)
+    h = dr.host
+    v3m, v3mtw = face_matrices(order)
+    c = collocation_coefficients(order, setup.dt)
+    ip, im = _np(dr.imposed_plus), _np(dr.imposed_minus)
+    # godunov_dr: interpolation of the random derivatives at the Gauss-Legendre points of [0, dt]
+    deriv0 = _np(layer.derivatives).astype(np.float64)
+    run_proxy(eng, layer, "godunov_dr", 1, setup.dt, dr=dr)
+    torch.cuda.synchronize()
+    rp, rm = oracle.dr_interpolate(order, dr.points, v3m, deriv0, h["plus"], h["minus"], h["face_info"],
+                                   h["tinv_t"], c)
+    e1 = max(rel_l2(_np(dr.q_plus), rp), rel_l2(_np(dr.q_minus), rm))
+    # neigh_dr: no regular face, the nodal flux of the imposed states
+    q0 = _np(layer.dofs).astype(np.float64)
+    run_proxy(eng, layer, "neigh_dr", 1, setup.dt, dr=dr)
+    torch.cuda.synchronize()
+    q1 = _np(layer.dofs).astype(np.float64)
+    ref1 = oracle.dr_nodal_flux(order, dr.points, v3mtw, h["dr_mapping"], h["plus"], h["face_info"], h["flux_plus"],
+                                h["flux_minus"], ip, im, q0)
+    e2 = rel_l2(q1 - q0, ref1 - q0)
+    # all_dr: ader + volume (no local flux on DR faces), interpolation of the new derivatives, nodal flux
+    run_proxy(eng, layer, "all_dr", 1, setup.dt, dr=dr)
+    torch.cuda.synchronize()
+    gm = global_matrices(order)["flat"]
+    ft = np.full((setup.n, 4), _capi.SDG_FACE_DYNAMIC_RUPTURE, np.uint8)
+    star, ap = _np(layer.star64), _np(layer.aplus64)
+    I, deriv = oracle.ader(order, gm, monomial_integrate(order, 0.0, setup.dt), q1, star, want_derivatives=True)
+    ref2 = oracle.local(order, gm, I, star, ap, q1, ft)
+    rp2, rm2 = oracle.dr_interpolate(order, dr.points, v3m, deriv, h["plus"], h["minus"], h["face_info"],
+                                     h["tinv_t"], c)
+    ref2 = oracle.dr_nodal_flux(order, dr.points, v3mtw, h["dr_mapping"], h["plus"], h["face_info"], h["flux_plus"],
+                                h["flux_minus"], ip, im, ref2)
+    e3 = rel_l2(_np(layer.dofs).astype(np.float64) - q1, ref2 - q1)
+    e4 = max(rel_l2(_np(dr.q_plus), rp2), rel_l2(_np(dr.q_minus), rm2))
+    print(f"godunov_dr {e1:.2e} neigh_dr {e2:.2e} all_dr dofs {e3:.2e} all_dr q {e4:.2e}")
+    assert max(e1, e2, e3, e4) < tol
+    with pytest.raises(ValueError):
+        run_proxy(eng, layer, "all_dr", 1, setup.dt)
