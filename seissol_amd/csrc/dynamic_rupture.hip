@@ -83,9 +83,10 @@ __global__ __launch_bounds__(kDrThreads) void kDrInterpolate(int64_t nf, const r
   const int P = L.points;
   for (int i = tid; i < 81; i += kDrThreads) sT[i] = static_cast<real>(L.tinv_t[f * 81 + i]);
   for (int side = 0; side < 2; ++side) {
-    const int64_t e = side == 0 ? L.plus_element[f] : L.minus_element[f];
-    const int faceSide = L.face_info[f * 3 + side];
-    const int h = side == 0 ? 0 : L.face_info[f * 3 + 2];
+    // wave-uniform face data (see kDrNodalFlux)
+    const int64_t e = __builtin_amdgcn_readfirstlane(side == 0 ? L.plus_element[f] : L.minus_element[f]);
+    const int faceSide = __builtin_amdgcn_readfirstlane(static_cast<int>(L.face_info[f * 3 + side]));
+    const int h = side == 0 ? 0 : __builtin_amdgcn_readfirstlane(static_cast<int>(L.face_info[f * 3 + 2]));
     const real* Vf = V + ((static_cast<int64_t>(min(faceSide, 3)) * 4 + min(h, 3)) * NP) * B;
     double* out = side == 0 ? outPlus : outMinus;
     for (int o = 0; o < O; ++o) {
@@ -151,8 +152,15 @@ __global__ __launch_bounds__(kDrThreads) void kDrNodalFlux(int64_t n, const real
     const int32_t m = drMap[e * 4 + face];
     if (m < 0 || m >= L.num_faces) continue;
     any = true;
-    const bool plus = L.plus_element[m] == e && L.face_info[m * 3 + 0] == face;
-    const int h = plus ? 0 : min(static_cast<int>(L.face_info[m * 3 + 2]), 3);
+    // the face's side data as wave-uniform values: the three selects below
+    // then stay scalar.  (With the byte loaded per lane, the compiler lowered
+    // `plus` as a divergent branch and kept the plus flux solver on the path
+    // where plus_element matches and the side does not.)
+    const int plusElem = __builtin_amdgcn_readfirstlane(L.plus_element[m]);
+    const int plusSide = __builtin_amdgcn_readfirstlane(static_cast<int>(L.face_info[m * 3 + 0]));
+    const int relation = __builtin_amdgcn_readfirstlane(static_cast<int>(L.face_info[m * 3 + 2]));
+    const bool plus = plusElem == e && plusSide == face;
+    const int h = plus ? 0 : min(relation, 3);
     const double* G = (plus ? impPlus : impMinus) + static_cast<int64_t>(m) * 9 * P;
     const double* F = (plus ? L.flux_solver_plus : L.flux_solver_minus) + static_cast<int64_t>(m) * 81;
     __syncthreads();

@@ -252,51 +252,60 @@ __device__ inline void gemmTile(floatx4 (&acc)[JJ][NQ], const uint16_t* const (&
 }
 
 // Y += sum_j (A_j X) star_j for the lane's element, with the matrix and
-// column scales divided out.  The elastic star matrix maps velocities (6..8)
-// to stresses (0..5) and stresses to velocities, so the contraction runs in
-// two passes of 3 and 6 input quantities, which bounds the live accumulators
-// at 3 x 6 tiles.
-template <int V, int LDX>
-__device__ inline void gemmStars(const uint16_t* const (&a)[3], int ksteps, const typename Prec<V>::T* xs, int lane,
+// column scales divided out.  The fragments of the three matrices for all
+// k-steps (ks <= KSMAX) are loaded once; the products then run in three
+// passes over input quantity groups {6,7,8}, {0,1,2}, {3,4,5}, each holding
+// 3 x 3 accumulator tiles: the elastic star matrix maps velocities (6..8) to
+// stresses (0..5) and stresses to velocities.
+template <int V, int KSMAX, int LDX>
+__device__ inline void gemmStars(const uint16_t* const (&a)[3], int ks, const typename Prec<V>::T* xs, int lane,
                                  const float* sS, const float (&aUn)[3], const float (&colUn)[9], float (&y)[9][4]) {
-  {
+  using P = Prec<V>;
+  constexpr int NX = P::NX;
+  const int g = lane >> 4, e = lane & 15;
+  typename P::V8 af[KSMAX][3][NX];
+#pragma unroll
+  for (int kk = 0; kk < KSMAX; ++kk)
+    if (kk < ks)
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+        for (int i = 0; i < NX; ++i) af[kk][jj][i] = loadFrag<V>(a[jj] + kk * kFragPair + i * kFragPlane, lane);
+#pragma unroll
+  for (int pass = 0; pass < 3; ++pass) {
+    const int q0 = pass == 0 ? 6 : (pass == 1 ? 0 : 3);
     floatx4 acc[3][3];
 #pragma unroll
     for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
       for (int i = 0; i < 3; ++i) acc[jj][i] = floatx4{0, 0, 0, 0};
-    gemmTile<V, 3, 6, 3, LDX>(acc, a, ksteps, xs, lane);
+#pragma unroll
+    for (int kk = 0; kk < KSMAX; ++kk)
+      if (kk < ks)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const int off = ((q0 + i) * 16 + e) * LDX + kk * 32 + 8 * g;
+          typename P::V8 bf[NX];
+#pragma unroll
+          for (int pl = 0; pl < NX; ++pl) bf[pl] = *reinterpret_cast<const typename P::V8*>(xs + pl * kCols * LDX + off);
+#pragma unroll
+          for (int jj = 0; jj < 3; ++jj) acc[jj][i] = mfmaSplit<V>(af[kk][jj], bf, acc[jj][i]);
+        }
+    // contraction with the star matrices for the outputs this group feeds
 #pragma unroll
     for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
-      for (int p = 0; p < 6; ++p)
+      for (int p = 0; p < 9; ++p)
 #pragma unroll
         for (int k = 0; k < 3; ++k)
           if (k < kStarCnt[p]) {
             const int qi = kStarQ[p][k];
-            const float sv = sS[jj * 24 + kStarOff[p] + k] * aUn[jj] * colUn[qi];
+            if (qi >= q0 && qi < q0 + 3) {
+              const float sv = sS[jj * 24 + kStarOff[p] + k] * aUn[jj] * colUn[qi];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) y[p][j] += acc[jj][qi - 6][j] * sv;
+              for (int j = 0; j < 4; ++j) y[p][j] += acc[jj][qi - q0][j] * sv;
+            }
           }
-  }
-  {
-    floatx4 acc[3][6];
-#pragma unroll
-    for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-      for (int i = 0; i < 6; ++i) acc[jj][i] = floatx4{0, 0, 0, 0};
-    gemmTile<V, 3, 0, 6, LDX>(acc, a, ksteps, xs, lane);
-#pragma unroll
-    for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-      for (int p = 6; p < 9; ++p)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const int qi = kStarQ[p][k];
-          const float sv = sS[jj * 24 + kStarOff[p] + k] * aUn[jj] * colUn[qi];
-#pragma unroll
-          for (int j = 0; j < 4; ++j) y[p][j] += acc[jj][qi][j] * sv;
-        }
   }
 }
 
@@ -317,11 +326,16 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
   __shared__ float sMax[scaled ? MT * kCols : 1];
 
   const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
+  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));  // wave-uniform
   const int g = lane >> 4, e = lane & 15;
   const int64_t E0 = static_cast<int64_t>(blockIdx.x) * kTileElems;
   const int64_t elem = E0 + e;
   const bool valid = elem < n;
+  // per-lane addresses as 32-bit offsets from uniform tile bases (one SGPR
+  // pair each) rather than 36 live 64-bit addresses
+  float* const tileDofs = dofs + E0 * 9 * B;
+  const int laneQ = e * 9 * B + 16 * w + 4 * g;
+  float* const tileTraces = traces + E0 * 36 * B2;
   const float* aUn = tables + S::tAUn;
   const float aUnCk[3] = {aUn[0], aUn[1], aUn[2]};
   const float aUnVol[3] = {aUn[3], aUn[4], aUn[5]};
@@ -332,7 +346,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     int z = r % 24, p = 0;
     while (z >= kStarCnt[p]) { z -= kStarCnt[p]; ++p; }
     const int q = kStarQ[p][z];
-    sStar[i] = (E0 + s < n) ? star[(E0 + s) * 243 + 81 * jj + q + 9 * p] : 0.0f;
+    sStar[i] = (E0 + s < n) ? star[E0 * 243 + s * 243 + 81 * jj + q + 9 * p] : 0.0f;
   }
 
   // Q itself is read again at the end, so only the time integral stays live
@@ -342,7 +356,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int row = 16 * w + 4 * g + j;
-      q[0][p][j] = (valid && row < B) ? dofs[elem * 9 * B + p * B + row] : 0.0f;
+      q[0][p][j] = (valid && row < B) ? tileDofs[laneQ + p * B + j] : 0.0f;
       iacc[0][p][j] = cf.c[0] * q[0][p][j];
     }
   // X <- Q
@@ -375,7 +389,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
       const uint16_t* a[3] = {frag + S::ck + (0LL * MT + w) * S::KSB * kFragPair,
                               frag + S::ck + (1LL * MT + w) * S::KSB * kFragPair,
                               frag + S::ck + (2LL * MT + w) * S::KSB * kFragPair};
-      gemmStars<V, LDX>(a, ks, xs, lane, sS, aUnCk, un, dq[0]);
+      gemmStars<V, S::KSB, LDX>(a, ks, xs, lane, sS, aUnCk, un, dq[0]);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         if (16 * w + 4 * g + j >= m)
@@ -401,12 +415,13 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
 
   // time-integrated DOFs: optional output, and the operand of the next GEMMs
   if (integ != nullptr && valid) {
+    float* const tileInteg = integ + E0 * 9 * B;
 #pragma unroll
     for (int p = 0; p < 9; ++p)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int row = 16 * w + 4 * g + j;
-        if (row < B) integ[elem * 9 * B + p * B + row] = iacc[0][p][j];
+        if (row < B) tileInteg[laneQ + p * B + j] = iacc[0][p][j];
       }
   }
   if constexpr (scaled) {
@@ -419,19 +434,6 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
   for (int t = MT + w; t < S::KPB / 16; t += MT)
     for (int p = 0; p < 9; ++p) storeZero4<V, LDX>(xs, p * 16 + e, 16 * t + 4 * g);
   __syncthreads();
-
-  // volume: Q += sum_j kDivM_j I star_j
-  float vol[9][4];
-#pragma unroll
-  for (int p = 0; p < 9; ++p)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) vol[p][j] = 0.0f;
-  {
-    const uint16_t* a[3] = {frag + S::vol + (0LL * MT + w) * S::KSB * kFragPair,
-                            frag + S::vol + (1LL * MT + w) * S::KSB * kFragPair,
-                            frag + S::vol + (2LL * MT + w) * S::KSB * kFragPair};
-    gemmStars<V, LDX>(a, S::KSB, xs, lane, sS, aUnVol, un, vol);
-  }
 
   // face traces T = fMrT_f I (rows f * B2 + m)
   float tr[TPW][9][4];
@@ -461,7 +463,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
             const int f = r / B2, mm = r % B2;
             const float s = tables[S::tInvM2 + mm];
 #pragma unroll
-            for (int p = 0; p < 9; ++p) traces[((elem * 4 + f) * 9 + p) * B2 + mm] = tr[ti][p][j] * s;
+            for (int p = 0; p < 9; ++p) tileTraces[(e * 36 + f * 9 + p) * B2 + mm] = tr[ti][p][j] * s;
           }
         }
       }
@@ -474,7 +476,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     __syncthreads();
     for (int i = threadIdx.x; i < kTileElems * 81; i += blockDim.x) {
       const int s = i / 81, r = i % 81;
-      sAp[i] = (E0 + s < n) ? aplus[(E0 + s) * 324 + f * 81 + r] : 0.0f;
+      sAp[i] = (E0 + s < n) ? aplus[E0 * 324 + s * 324 + f * 81 + r] : 0.0f;
     }
     __syncthreads();
     const bool dr = valid && faceTypes != nullptr && faceTypes[elem * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE;
@@ -500,6 +502,20 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
       }
     }
   }
+  // volume: Q += sum_j kDivM_j I star_j (X still holds I; computed after W so
+  // that its result is not live across the face products)
+  float vol[9][4];
+#pragma unroll
+  for (int p = 0; p < 9; ++p)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) vol[p][j] = 0.0f;
+  {
+    const uint16_t* a[3] = {frag + S::vol + (0LL * MT + w) * S::KSB * kFragPair,
+                            frag + S::vol + (1LL * MT + w) * S::KSB * kFragPair,
+                            frag + S::vol + (2LL * MT + w) * S::KSB * kFragPair};
+    gemmStars<V, S::KSB, LDX>(a, S::KSB, xs, lane, sS, aUnVol, un, vol);
+  }
+
   if constexpr (scaled) waveColumnMax<TPW>(tr, true, sMax, w, lane);
   __syncthreads();
   if constexpr (scaled) columnScales<MT>(sMax, e, cs, un);
@@ -528,7 +544,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
         for (int j = 0; j < 4; ++j) {
           const int row = 16 * w + 4 * g + j;
           if (row < B) {
-            float* d = dofs + elem * 9 * B + p * B + row;
+            float* d = tileDofs + laneQ + p * B + j;
             *d = *d + (vol[p][j] + acc[0][p][j] * (aUnR * un[p]));
           }
         }
@@ -554,11 +570,13 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   __shared__ unsigned sMaxU[scaled ? kCols : 1];
 
   const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
+  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));  // wave-uniform
   const int g = lane >> 4, e = lane & 15;
   const int64_t E0 = static_cast<int64_t>(blockIdx.x) * kTileElems;
   const int64_t elem = E0 + e;
   const bool valid = elem < n;
+  float* const tileDofs = dofs + E0 * 9 * B;
+  const int laneQ = e * 9 * B + 16 * w + 4 * g;
 
   for (int i = threadIdx.x; i < 3 * B2 * B2; i += blockDim.x) sFP[i] = tables[S::tFP + i];
   if constexpr (scaled)
@@ -587,7 +605,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
     }
     for (int i = threadIdx.x; i < kTileElems * 81; i += blockDim.x) {
       const int s = i / 81, r = i % 81;
-      sAm[i] = (E0 + s < n) ? aminus[(E0 + s) * 324 + f * 81 + r] : 0.0f;
+      sAm[i] = (E0 + s < n) ? aminus[E0 * 324 + s * 324 + f * 81 + r] : 0.0f;
     }
     __syncthreads();
 #pragma unroll
@@ -664,7 +682,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int row = 16 * w + 4 * g + j;
-        if (row < B) dofs[elem * 9 * B + p * B + row] += acc[0][p][j] * (aUnR * un[p]);
+        if (row < B) tileDofs[laneQ + p * B + j] += acc[0][p][j] * (aUnR * un[p]);
       }
   }
 }
