@@ -251,6 +251,54 @@ __device__ inline void gemmTile(floatx4 (&acc)[JJ][NQ], const uint16_t* const (&
   }
 }
 
+// The lane's four consecutive basis rows row0..row0+3 of the 9 quantities of
+// its element at q[p * B + j] (row0 = 16 w + 4 g): one 16-byte access per
+// quantity when B is a multiple of 4 (rows 16-byte aligned), else per value.
+template <int B>
+__device__ inline void loadRows(const float* q, int row0, bool valid, float (&v)[9][4]) {
+  if constexpr (B % 4 == 0) {
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+      const float4 x = (valid && row0 < B) ? *reinterpret_cast<const float4*>(q + p * B) : float4{0, 0, 0, 0};
+      v[p][0] = x.x;
+      v[p][1] = x.y;
+      v[p][2] = x.z;
+      v[p][3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[p][j] = (valid && row0 + j < B) ? q[p * B + j] : 0.0f;
+  }
+}
+
+// q[p * B + j] += (add ? add[p][j] : 0) + acc[p][j] * s * un[p] for the lane's rows
+template <int B>
+__device__ inline void updateRows(float* q, int row0, const floatx4 (&acc)[9], float s, const float (&un)[9],
+                                  const float (*add)[4]) {
+  if constexpr (B % 4 == 0) {
+    if (row0 < B)
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        float4* d = reinterpret_cast<float4*>(q + p * B);
+        float4 x = *d;
+        const float sc = s * un[p];
+        x.x += (add ? add[p][0] : 0.0f) + acc[p][0] * sc;
+        x.y += (add ? add[p][1] : 0.0f) + acc[p][1] * sc;
+        x.z += (add ? add[p][2] : 0.0f) + acc[p][2] * sc;
+        x.w += (add ? add[p][3] : 0.0f) + acc[p][3] * sc;
+        *d = x;
+      }
+  } else {
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (row0 + j < B) q[p * B + j] += (add ? add[p][j] : 0.0f) + acc[p][j] * (s * un[p]);
+  }
+}
+
 // Y += sum_j (A_j X) star_j for the lane's element, with the matrix and
 // column scales divided out.  The fragments of the three matrices for all
 // k-steps (ks <= KSMAX) are loaded once; the products then run in three
@@ -351,14 +399,11 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
 
   // Q itself is read again at the end, so only the time integral stays live
   float q[1][9][4], iacc[1][9][4];
+  loadRows<B>(tileDofs + laneQ, 16 * w + 4 * g, valid, q[0]);
 #pragma unroll
   for (int p = 0; p < 9; ++p)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int row = 16 * w + 4 * g + j;
-      q[0][p][j] = (valid && row < B) ? tileDofs[laneQ + p * B + j] : 0.0f;
-      iacc[0][p][j] = cf.c[0] * q[0][p][j];
-    }
+    for (int j = 0; j < 4; ++j) iacc[0][p][j] = cf.c[0] * q[0][p][j];
   // X <- Q
   float cs[9], un[9];
 #pragma unroll
@@ -435,6 +480,19 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     for (int p = 0; p < 9; ++p) storeZero4<V, LDX>(xs, p * 16 + e, 16 * t + 4 * g);
   __syncthreads();
 
+  // AplusT of face 0 in flight during the trace GEMM; face f + 1 while face f is applied
+  constexpr int APT = cdiv(kTileElems * 81, S::THREADS);
+  float apv[APT];
+  auto loadAplus = [&](int f) {
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int idx = threadIdx.x + i * S::THREADS;
+      apv[i] = (idx < kTileElems * 81 && E0 + idx / 81 < n) ? aplus[E0 * 324 + (idx / 81) * 324 + f * 81 + idx % 81]
+                                                             : 0.0f;
+    }
+  };
+  loadAplus(0);
+
   // face traces T = fMrT_f I (rows f * B2 + m)
   float tr[TPW][9][4];
 #pragma unroll
@@ -474,10 +532,12 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
 #pragma unroll 1
   for (int f = 0; f < 4; ++f) {
     __syncthreads();
-    for (int i = threadIdx.x; i < kTileElems * 81; i += blockDim.x) {
-      const int s = i / 81, r = i % 81;
-      sAp[i] = (E0 + s < n) ? aplus[E0 * 324 + s * 324 + f * 81 + r] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int idx = threadIdx.x + i * S::THREADS;
+      if (idx < kTileElems * 81) sAp[idx] = apv[i];
     }
+    if (f < 3) loadAplus(f + 1);
     __syncthreads();
     const bool dr = valid && faceTypes != nullptr && faceTypes[elem * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE;
     const float* A = sAp + e * 81;
@@ -537,18 +597,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     for (int p = 0; p < 9; ++p) acc[0][p] = floatx4{0, 0, 0, 0};
     const uint16_t* a[1] = {frag + S::rdm + static_cast<int64_t>(w) * S::KSF * kFragPair};
     gemmTile<V, 1, 0, 9, LDX>(acc, a, S::KSF, xs, lane);
-    if (valid) {
-#pragma unroll
-      for (int p = 0; p < 9; ++p)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int row = 16 * w + 4 * g + j;
-          if (row < B) {
-            float* d = tileDofs + laneQ + p * B + j;
-            *d = *d + (vol[p][j] + acc[0][p][j] * (aUnR * un[p]));
-          }
-        }
-    }
+    if (valid) updateRows<B>(tileDofs + laneQ, 16 * w + 4 * g, acc[0], aUnR, un, vol);
   }
 }
 
@@ -568,6 +617,11 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   __shared__ float sAm[kTileElems * 81];
   __shared__ float sFP[3 * B2 * B2];
   __shared__ unsigned sMaxU[scaled ? kCols : 1];
+  __shared__ int sLink[kTileElems * 4];  // trace index nb * 4 + side of (element, face), -1: none
+  __shared__ int sRel[kTileElems * 4];   // orientation h of (element, face)
+  constexpr int TR = 9 * B2;                                   // floats of one face trace
+  constexpr int GPT = cdiv(kTileElems * TR, S::THREADS);       // gathered trace floats per thread
+  constexpr int APT = cdiv(kTileElems * 81, S::THREADS);       // AminusT floats per thread
 
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));  // wave-uniform
@@ -579,6 +633,18 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   const int laneQ = e * 9 * B + 16 * w + 4 * g;
 
   for (int i = threadIdx.x; i < 3 * B2 * B2; i += blockDim.x) sFP[i] = tables[S::tFP + i];
+  // neighbour links of the four faces, read once
+  if (threadIdx.x < kTileElems * 4) {
+    const int s = threadIdx.x >> 2, f = threadIdx.x & 3;
+    const int64_t el = E0 + s;
+    int link = -1, h = 0;
+    if (el < n && (faceTypes == nullptr || faceTypes[el * 4 + f] == SDG_FACE_REGULAR)) {
+      link = neighbors[el * 4 + f] * 4 + min(static_cast<int>(faceRel[(el * 4 + f) * 2 + 0]), 3);
+      h = min(static_cast<int>(faceRel[(el * 4 + f) * 2 + 1]), 2);
+    }
+    sLink[threadIdx.x] = link;
+    sRel[threadIdx.x] = h;
+  }
   if constexpr (scaled)
     for (int i = threadIdx.x; i < kCols; i += blockDim.x) sMaxU[i] = 0u;
   // zero padding rows of the face operand
@@ -587,35 +653,53 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
 #pragma unroll
     for (int pl = 0; pl < NX; ++pl) xs[pl * kCols * LDX + col * LDX + k] = static_cast<T>(0.0f);
   }
+  __syncthreads();
+  // the next face's traces and AminusT are loaded into registers while the
+  // current face is computed
+  float gv[GPT], av[APT];
+  auto gather = [&](int f) {
+#pragma unroll
+    for (int i = 0; i < GPT; ++i) {
+      const int idx = threadIdx.x + i * S::THREADS;
+      float v = 0.0f;
+      if (idx < kTileElems * TR) {
+        const int link = sLink[(idx / TR) * 4 + f];
+        if (link >= 0) v = traces[static_cast<int64_t>(link) * TR + idx % TR];
+      }
+      gv[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int idx = threadIdx.x + i * S::THREADS;
+      av[i] = (idx < kTileElems * 81 && E0 + idx / 81 < n) ? aminus[E0 * 324 + (idx / 81) * 324 + f * 81 + idx % 81]
+                                                            : 0.0f;
+    }
+  };
+  gather(0);
   // the fp16 form keeps the face operand in registers until its column scales are known
   float wv[scaled ? 4 : 1][scaled ? IPT : 1][9];
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
     __syncthreads();
-    for (int i = threadIdx.x; i < kTileElems * 9 * B2; i += blockDim.x) {
-      const int s = i / (9 * B2), r = i % (9 * B2);
-      const int64_t el = E0 + s;
-      float v = 0.0f;
-      if (el < n && (faceTypes == nullptr || faceTypes[el * 4 + f] == SDG_FACE_REGULAR)) {
-        const int64_t nb = neighbors[el * 4 + f];
-        const int side = min(static_cast<int>(faceRel[(el * 4 + f) * 2 + 0]), 3);
-        v = traces[(nb * 4 + side) * 9 * B2 + r];
-      }
-      sTr[i] = v;
+#pragma unroll
+    for (int i = 0; i < GPT; ++i) {
+      const int idx = threadIdx.x + i * S::THREADS;
+      if (idx < kTileElems * TR) sTr[idx] = gv[i];
     }
-    for (int i = threadIdx.x; i < kTileElems * 81; i += blockDim.x) {
-      const int s = i / 81, r = i % 81;
-      sAm[i] = (E0 + s < n) ? aminus[E0 * 324 + s * 324 + f * 81 + r] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int idx = threadIdx.x + i * S::THREADS;
+      if (idx < kTileElems * 81) sAm[idx] = av[i];
     }
+    if (f < 3) gather(f + 1);
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < IPT; ++it) {
       const int item = threadIdx.x + it * S::THREADS;
       if (item < ITEMS) {
         const int s = item / B2, row = item % B2;
-        const int64_t el = E0 + s;
-        const bool regular = el < n && (faceTypes == nullptr || faceTypes[el * 4 + f] == SDG_FACE_REGULAR);
-        const int h = regular ? min(static_cast<int>(faceRel[(el * 4 + f) * 2 + 1]), 2) : 0;
+        const bool regular = sLink[s * 4 + f] >= 0;
+        const int h = sRel[s * 4 + f];
         float u[9];
 #pragma unroll
         for (int qq = 0; qq < 9; ++qq) u[qq] = 0.0f;
@@ -676,15 +760,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   const uint16_t* a[1] = {frag + S::rdm + static_cast<int64_t>(w) * S::KSF * kFragPair};
   gemmTile<V, 1, 0, 9, LDX>(acc, a, S::KSF, xs, lane);
   const float aUnR = tables[S::tAUn + 7];
-  if (valid) {
-#pragma unroll
-    for (int p = 0; p < 9; ++p)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int row = 16 * w + 4 * g + j;
-        if (row < B) tileDofs[laneQ + p * B + j] += acc[0][p][j] * (aUnR * un[p]);
-      }
-  }
+  if (valid) updateRows<B>(tileDofs + laneQ, 16 * w + 4 * g, acc[0], aUnR, un, nullptr);
 }
 
 template <int O, int V>
