@@ -112,7 +112,9 @@ typedef struct {
   const int32_t* face_neighbors;  /* [n][4], index into time_integrated       */
   const uint8_t* face_relations;  /* [n][4][2]                                */
   const uint8_t* face_types;      /* [n][4] or NULL (all regular)             */
-  void* face_traces;              /* [n][4][9][B2]: rT_f I of every face;     */
+  void* face_traces;              /* [n][4][9][B2P] FP32, B2P = B2 rounded up */
+                                  /* to a multiple of 4 (padding rows zero):  */
+                                  /* rT_f I of every face;                    */
                                   /* required by the matrix-core arithmetics  */
                                   /* SDG_BF16X3/X6, SDG_F16X3 (written by the */
                                   /* local pass, read by the neighbour pass)  */

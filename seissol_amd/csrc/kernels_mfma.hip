@@ -98,7 +98,7 @@ struct Cfg {
   static constexpr int WAVES_L = (V != 3 && LDS_L <= 81920) ? 2 : 1;
   static constexpr int LDX_N0 = S::KMAX + 8;
   static constexpr int lds_n(int ldx) {
-    return NX * kCols * ldx * 2 + kTileElems * 9 * S::B2 * 4 + kTileElems * 81 * 4 + 3 * S::B2 * S::B2 * 4 +
+    return NX * kCols * ldx * 2 + kTileElems * 9 * S::B2P * 4 + kTileElems * 81 * 4 + 3 * S::B2 * S::B2 * 4 +
            (scaled ? kCols * 4 : 0);
   }
   // neighbour kernel: drop the padding when that makes two workgroups fit
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
   // pair each) rather than 36 live 64-bit addresses
   float* const tileDofs = dofs + E0 * 9 * B;
   const int laneQ = e * 9 * B + 16 * w + 4 * g;
-  float* const tileTraces = traces + E0 * 36 * B2;
+  float* const tileTraces = traces + E0 * 36 * S::B2P;
   const float* aUn = tables + S::tAUn;
   const float aUnCk[3] = {aUn[0], aUn[1], aUn[2]};
   const float aUnVol[3] = {aUn[3], aUn[4], aUn[5]};
@@ -512,18 +512,19 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
       for (int p = 0; p < 9; ++p)
 #pragma unroll
         for (int j = 0; j < 4; ++j) tr[ti][p][j] = acc[0][p][j] * (aUnF * un[p]);
-      // neighbour-side traces rT_f I = T / M2
-      if (valid) {
+      // neighbour-side traces rT_f I = T / M2: the lane's four rows lie in one
+      // face (faces padded to B2P rows, a multiple of 4), one 16-byte store
+      // per quantity into the padded layout [element][face][quantity][B2P]
+      const int r0 = 16 * t + 4 * g;
+      if (valid && r0 < FR) {
+        const int f = r0 / S::B2P, mm = r0 % S::B2P;
+        float sc[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int r = 16 * t + 4 * g + j;
-          if (r < FR) {
-            const int f = r / B2, mm = r % B2;
-            const float s = tables[S::tInvM2 + mm];
+        for (int j = 0; j < 4; ++j) sc[j] = mm + j < B2 ? tables[S::tInvM2 + mm + j] : 0.0f;
 #pragma unroll
-            for (int p = 0; p < 9; ++p) tileTraces[(e * 36 + f * 9 + p) * B2 + mm] = tr[ti][p][j] * s;
-          }
-        }
+        for (int p = 0; p < 9; ++p)
+          *reinterpret_cast<float4*>(tileTraces + (e * 36 + f * 9 + p) * S::B2P + mm) =
+              float4{tr[ti][p][0] * sc[0], tr[ti][p][1] * sc[1], tr[ti][p][2] * sc[2], tr[ti][p][3] * sc[3]};
       }
     }
   }
@@ -547,7 +548,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int r = 16 * t + 4 * g + j;
-        if (t < S::MTF && r < FR && r / B2 == f) {
+        if (t < S::MTF && r < FR && r / S::B2P == f) {
           float wrow[9];
 #pragma unroll
           for (int p = 0; p < 9; ++p) {
@@ -613,14 +614,15 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   constexpr bool scaled = C::scaled;
   constexpr int ITEMS = kTileElems * B2;
   __shared__ T xs[NX * kCols * LDX];
-  __shared__ float sTr[kTileElems * 9 * B2];
+  __shared__ __attribute__((aligned(16))) float sTr[kTileElems * 9 * S::B2P];
   __shared__ float sAm[kTileElems * 81];
   __shared__ float sFP[3 * B2 * B2];
   __shared__ unsigned sMaxU[scaled ? kCols : 1];
   __shared__ int sLink[kTileElems * 4];  // trace index nb * 4 + side of (element, face), -1: none
   __shared__ int sRel[kTileElems * 4];   // orientation h of (element, face)
-  constexpr int TR = 9 * B2;                                   // floats of one face trace
-  constexpr int GPT = cdiv(kTileElems * TR, S::THREADS);       // gathered trace floats per thread
+  constexpr int TR = 9 * S::B2P;                               // floats of one (padded) face trace
+  constexpr int TR4 = TR / 4;                                  // 16-byte pieces of one face trace
+  constexpr int GPT = cdiv(kTileElems * TR4, S::THREADS);      // gathered pieces per thread
   constexpr int APT = cdiv(kTileElems * 81, S::THREADS);       // AminusT floats per thread
 
   const int lane = threadIdx.x & 63;
@@ -648,23 +650,25 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   if constexpr (scaled)
     for (int i = threadIdx.x; i < kCols; i += blockDim.x) sMaxU[i] = 0u;
   // zero padding rows of the face operand
-  for (int i = threadIdx.x; i < kCols * (S::KPF - FR); i += blockDim.x) {
-    const int col = i / (S::KPF - FR), k = FR + i % (S::KPF - FR);
+  for (int i = threadIdx.x; i < kCols * S::KPF; i += blockDim.x) {
+    const int col = i / S::KPF, k = i % S::KPF;
+    if (k >= FR || k % S::B2P >= B2)
 #pragma unroll
-    for (int pl = 0; pl < NX; ++pl) xs[pl * kCols * LDX + col * LDX + k] = static_cast<T>(0.0f);
+      for (int pl = 0; pl < NX; ++pl) xs[pl * kCols * LDX + col * LDX + k] = static_cast<T>(0.0f);
   }
   __syncthreads();
   // the next face's traces and AminusT are loaded into registers while the
   // current face is computed
-  float gv[GPT], av[APT];
+  float4 gv[GPT];
+  float av[APT];
   auto gather = [&](int f) {
 #pragma unroll
     for (int i = 0; i < GPT; ++i) {
       const int idx = threadIdx.x + i * S::THREADS;
-      float v = 0.0f;
-      if (idx < kTileElems * TR) {
-        const int link = sLink[(idx / TR) * 4 + f];
-        if (link >= 0) v = traces[static_cast<int64_t>(link) * TR + idx % TR];
+      float4 v{0.0f, 0.0f, 0.0f, 0.0f};
+      if (idx < kTileElems * TR4) {
+        const int link = sLink[(idx / TR4) * 4 + f];
+        if (link >= 0) v = *reinterpret_cast<const float4*>(traces + static_cast<int64_t>(link) * TR + 4 * (idx % TR4));
       }
       gv[i] = v;
     }
@@ -684,7 +688,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
 #pragma unroll
     for (int i = 0; i < GPT; ++i) {
       const int idx = threadIdx.x + i * S::THREADS;
-      if (idx < kTileElems * TR) sTr[idx] = gv[i];
+      if (idx < kTileElems * TR4) *reinterpret_cast<float4*>(sTr + 4 * idx) = gv[i];
     }
 #pragma unroll
     for (int i = 0; i < APT; ++i) {
@@ -704,11 +708,11 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
 #pragma unroll
         for (int qq = 0; qq < 9; ++qq) u[qq] = 0.0f;
         const float* fp = sFP + (h * B2 + row) * B2;
-        const float* tr = sTr + s * 9 * B2;
+        const float* tr = sTr + s * TR;
         for (int mm = 0; mm < B2; ++mm) {
           const float c = fp[mm];
 #pragma unroll
-          for (int qq = 0; qq < 9; ++qq) u[qq] += c * tr[qq * B2 + mm];
+          for (int qq = 0; qq < 9; ++qq) u[qq] += c * tr[qq * S::B2P + mm];
         }
         const float* A = sAm + s * 81;
 #pragma unroll
@@ -721,7 +725,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
             wv[f][it][p] = acc;
             atomicMax(&sMaxU[p * 16 + s], __float_as_uint(fabsf(acc)));
           } else {
-            storeX1<V, LDX>(xs, p * 16 + s, f * B2 + row, acc);
+            storeX1<V, LDX>(xs, p * 16 + s, f * S::B2P + row, acc);
           }
         }
       }
@@ -743,7 +747,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
           for (int p = 0; p < 9; ++p) {
             float sc, inv;
             columnScale(__uint_as_float(sMaxU[p * 16 + s]), sc, inv);
-            storeX1<V, LDX>(xs, p * 16 + s, f * B2 + row, wv[f][it][p] * sc);
+            storeX1<V, LDX>(xs, p * 16 + s, f * S::B2P + row, wv[f][it][p] * sc);
           }
         }
       }

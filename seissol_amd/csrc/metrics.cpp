@@ -131,7 +131,8 @@ extern "C" int sdg_performance_estimate(int order, int arithmetic, int kernel_id
   const double s = arithmetic == SDG_FP64 ? 8.0 : 4.0;
   const double products = arithmetic == SDG_BF16X6 ? 6.0 : (matrixCore(arithmetic) ? 3.0 : 1.0);
   const bool traces = matrixCore(arithmetic);
-  const double q = 9.0 * B, traceBytes = 4.0 * 9 * B2 * 4;
+  const int B2P = (B2 + 3) / 4 * 4;  // padded face rows of the matrix-core traces
+  const double q = 9.0 * B, traceBytes = 4.0 * 9 * B2P * 4;
   Counts c;
   double bytes = 0, kernelBytes = 0;
   auto add = [&](const Counts& x) {

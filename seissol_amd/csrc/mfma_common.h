@@ -24,7 +24,8 @@ struct MShape {
   static constexpr int MT = cdiv(B, 16);            // row tiles of the basis index
   static constexpr int KPB = cdiv(B, 32) * 32;      // padded K over the basis index
   static constexpr int KSB = KPB / 32;
-  static constexpr int FR = 4 * B2;                 // rows of the four face operands
+  static constexpr int B2P = (B2 + 3) / 4 * 4;      // face rows padded to a multiple of 4
+  static constexpr int FR = 4 * B2P;                // rows of the four stacked (padded) face operands
   static constexpr int MTF = cdiv(FR, 16);
   static constexpr int KPF = cdiv(FR, 32) * 32;
   static constexpr int KSF = KPF / 32;

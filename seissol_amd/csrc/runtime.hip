@@ -148,10 +148,12 @@ int uploadMfmaData(EngineState& s, const double* gm) {
     add([&](int r, int c) { return gm[g.kDivM + (j * B + r) * B + c]; }, B, B, S::MT, S::KSB,
         S::vol + static_cast<int64_t>(j) * S::MT * S::KSB * mfma::kFragPair, 3 + j);
   }
-  add([&](int r, int c) { return gm[g.fMrT + ((r / B2) * B2 + r % B2) * B + c]; }, S::FR, B, S::MTF, S::KSB,
-      S::fmrt, 6);
-  add([&](int r, int c) { return gm[g.rDivM + ((c / B2) * B + r) * B2 + c % B2]; }, B, S::FR, S::MT, S::KSF,
-      S::rdm, 7);
+  // the four face operands stacked with B2P rows each (rows B2..B2P-1 zero)
+  constexpr int B2P = S::B2P;
+  add([&](int r, int c) { return r % B2P < B2 ? gm[g.fMrT + ((r / B2P) * B2 + r % B2P) * B + c] : 0.0; }, S::FR, B,
+      S::MTF, S::KSB, S::fmrt, 6);
+  add([&](int r, int c) { return c % B2P < B2 ? gm[g.rDivM + ((c / B2P) * B + r) * B2 + c % B2P] : 0.0; }, B, S::FR,
+      S::MT, S::KSF, S::rdm, 7);
   for (int m = 0; m < B2; ++m) tables[S::tInvM2 + m] = static_cast<float>(1.0 / gm[g.mass2 + m]);
   for (int i = 0; i < 3 * B2 * B2; ++i) tables[S::tFP + i] = static_cast<float>(gm[g.fP + i]);
   if (s.mfmaFrags == nullptr) SDG_HIP_CHECK(hipMalloc(&s.mfmaFrags, frags.size() * sizeof(uint16_t)));

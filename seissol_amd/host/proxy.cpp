@@ -225,7 +225,8 @@ int run(int argc, char** argv) {
   void* dInt = deviceZeros(n * 9 * B * real);
   const bool traces = arithmetic == Arithmetic::BF16x3 || arithmetic == Arithmetic::BF16x6 ||
                       arithmetic == Arithmetic::F16x3;
-  void* dTraces = traces ? deviceZeros(n * 4 * 9 * B2 * 4) : nullptr;
+  const int B2P = (B2 + 3) / 4 * 4;  // face rows of the traces, padded to a multiple of 4
+  void* dTraces = traces ? deviceZeros(n * 4 * 9 * B2P * 4) : nullptr;
   // dynamic rupture: random DR faces, derivatives and imposed states
   void* dDeriv = nullptr;
   std::vector<void*> drBuffers;

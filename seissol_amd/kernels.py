@@ -33,6 +33,11 @@ def num_face_basis(order: int) -> int:
     return order * (order + 1) // 2
 
 
+def padded_face_basis(order: int) -> int:
+    """Face rows of the matrix-core face traces: B2 rounded up to a multiple of 4."""
+    return (num_face_basis(order) + 3) // 4 * 4
+
+
 def derivatives_size(order: int) -> int:
     return int(lib().sdg_derivatives_size(order))
 

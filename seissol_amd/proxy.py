@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _capi
 from ._capi import DrLayer, Layer, ProxyDr, check, lib, ptr
-from .kernels import MATRIX_CORE, Engine, derivatives_size, num_basis, num_face_basis
+from .kernels import MATRIX_CORE, Engine, derivatives_size, num_basis, padded_face_basis
 
 KERNELS = ("all", "local", "neigh", "ader", "localwoader", "neigh_dr", "godunov_dr", "all_dr")
 DR_KERNELS = ("neigh_dr", "godunov_dr", "all_dr")
@@ -133,8 +133,8 @@ class DeviceLayer:
         self.time_integrated = torch.zeros((n, 9, B), dtype=rt, device=dev)
         self.derivatives = (torch.zeros((n, derivatives_size(setup.order)), dtype=rt, device=dev)
                             if with_derivatives else None)
-        B2 = num_face_basis(setup.order)
-        self.face_traces = (torch.zeros((n, 4, 9, B2), dtype=torch.float32, device=dev)
+        B2P = padded_face_basis(setup.order)
+        self.face_traces = (torch.zeros((n, 4, 9, B2P), dtype=torch.float32, device=dev)
                             if engine.arithmetic in MATRIX_CORE else None)
         self.view = Layer(n, ptr(self.dofs), ptr(self.time_integrated), ptr(self.derivatives), ptr(self.star),
                           ptr(self.aplus), ptr(self.aminus), ptr(self.neighbors), ptr(self.face_relations),
