@@ -300,25 +300,17 @@ __device__ inline void updateRows(float* q, int row0, const floatx4 (&acc)[9], f
 }
 
 // Y += sum_j (A_j X) star_j for the lane's element, with the matrix and
-// column scales divided out.  The fragments of the three matrices for all
-// k-steps (ks <= KSMAX) are loaded once; the products then run in three
-// passes over input quantity groups {6,7,8}, {0,1,2}, {3,4,5}, each holding
-// 3 x 3 accumulator tiles: the elastic star matrix maps velocities (6..8) to
-// stresses (0..5) and stresses to velocities.
+// column scales divided out.  The products run in three passes over input
+// quantity groups {6,7,8}, {0,1,2}, {3,4,5}, each holding 3 x 3 accumulator
+// tiles (the elastic star matrix maps velocities (6..8) to stresses (0..5)
+// and stresses to velocities); the three matrices' fragments of a k-step are
+// loaded inside the pass (L2-resident), which keeps them out of the live set.
 template <int V, int KSMAX, int LDX>
 __device__ inline void gemmStars(const uint16_t* const (&a)[3], int ks, const typename Prec<V>::T* xs, int lane,
                                  const float* sS, const float (&aUn)[3], const float (&colUn)[9], float (&y)[9][4]) {
   using P = Prec<V>;
   constexpr int NX = P::NX;
   const int g = lane >> 4, e = lane & 15;
-  typename P::V8 af[KSMAX][3][NX];
-#pragma unroll
-  for (int kk = 0; kk < KSMAX; ++kk)
-    if (kk < ks)
-#pragma unroll
-      for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-        for (int i = 0; i < NX; ++i) af[kk][jj][i] = loadFrag<V>(a[jj] + kk * kFragPair + i * kFragPlane, lane);
 #pragma unroll
   for (int pass = 0; pass < 3; ++pass) {
     const int q0 = pass == 0 ? 6 : (pass == 1 ? 0 : 3);
@@ -329,7 +321,12 @@ __device__ inline void gemmStars(const uint16_t* const (&a)[3], int ks, const ty
       for (int i = 0; i < 3; ++i) acc[jj][i] = floatx4{0, 0, 0, 0};
 #pragma unroll
     for (int kk = 0; kk < KSMAX; ++kk)
-      if (kk < ks)
+      if (kk < ks) {
+        typename P::V8 af[1][3][NX];
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+          for (int i = 0; i < NX; ++i) af[0][jj][i] = loadFrag<V>(a[jj] + kk * kFragPair + i * kFragPlane, lane);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
           const int off = ((q0 + i) * 16 + e) * LDX + kk * 32 + 8 * g;
@@ -337,8 +334,9 @@ __device__ inline void gemmStars(const uint16_t* const (&a)[3], int ks, const ty
 #pragma unroll
           for (int pl = 0; pl < NX; ++pl) bf[pl] = *reinterpret_cast<const typename P::V8*>(xs + pl * kCols * LDX + off);
 #pragma unroll
-          for (int jj = 0; jj < 3; ++jj) acc[jj][i] = mfmaSplit<V>(af[kk][jj], bf, acc[jj][i]);
+          for (int jj = 0; jj < 3; ++jj) acc[jj][i] = mfmaSplit<V>(af[0][jj], bf, acc[jj][i]);
         }
+      }
     // contraction with the star matrices for the outputs this group feeds
 #pragma unroll
     for (int jj = 0; jj < 3; ++jj)
@@ -591,14 +589,20 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     for (int p = 0; p < 9; ++p) storeZero4<V, LDX>(xs, p * 16 + e, 16 * t + 4 * g);
   __syncthreads();
 
-  // local flux: Q += [rDivM_0 .. rDivM_3] W
+  // local flux: Q += [rDivM_0 .. rDivM_3] W, the accumulators starting from the
+  // volume term divided by the (power-of-two) unscale factors, so that the
+  // update needs no separately live volume registers
   {
     floatx4 acc[1][9];
 #pragma unroll
-    for (int p = 0; p < 9; ++p) acc[0][p] = floatx4{0, 0, 0, 0};
+    for (int p = 0; p < 9; ++p) {
+      const float inv = 1.0f / (aUnR * un[p]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[0][p][j] = vol[p][j] * inv;
+    }
     const uint16_t* a[1] = {frag + S::rdm + static_cast<int64_t>(w) * S::KSF * kFragPair};
     gemmTile<V, 1, 0, 9, LDX>(acc, a, S::KSF, xs, lane);
-    if (valid) updateRows<B>(tileDofs + laneQ, 16 * w + 4 * g, acc[0], aUnR, un, vol);
+    if (valid) updateRows<B>(tileDofs + laneQ, 16 * w + 4 * g, acc[0], aUnR, un, nullptr);
   }
 }
 
