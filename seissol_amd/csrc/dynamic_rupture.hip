@@ -4,7 +4,7 @@
 //    (src/Initializer/Model/DynamicRuptureMatrices.cpp:150-470);
 //  * kDrInterpolate: space-time interpolation of both sides' derivatives at
 //    the Gauss-Legendre time points and the face quadrature points, rotated to
-//    the face frame (DynamicRupture::spaceTimeInterpolation,
+//    the face frame, batched by face-matrix variant (DynamicRupture::spaceTimeInterpolation,
 //    src/Kernels/DynamicRupture.cpp:57-168; spec dynamic_rupture.py:108-170);
 //  * kDrNodalFlux: the flux of the imposed states into the adjacent cells
 //    (Neighbor.cpp:93-101 with the nodalFlux spec, dynamic_rupture.py:174-190).
@@ -65,66 +65,114 @@ int ensureDrMatrices(EngineState& s) {
   return s.arithmetic == SDG_FP64 ? uploadDrMatrices<double>(s) : uploadDrMatrices<float>(s);
 }
 
-// One workgroup per DR face: both sides, all time points.  out[f][o][p][k]
-// (FP64, the friction law's q_interpolated layout) = sum_q (sum_l V[k][l]
-// Q_o[q][l]) TinvT(q, p), Q_o = sum_d c[o][d] dQ_d of the side's cell.
+constexpr int kDrChunk = 256;  // faces scanned by one interpolation workgroup (one per thread)
+constexpr int kDrWaves = 4;   // sides in flight per workgroup (one per wave)
+
+__device__ inline void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Space-time interpolation, batched by the 16 variants of V3mTo2n(i, h)
+// (h = 0: plus side, h = 1..3: minus side with that orientation), as the
+// reference batches per (side, faceRelation): workgroup (chunk, variant)
+// stages its matrix in LDS once and serves every side of its variant among
+// the chunk's kDrChunk faces, one side per wave.  Per side the derivatives
+// are interpolated first, w_d[k][q] = sum_l V[k][l] dQ_d[q][l] (lane k =
+// Stroud point), then the time points Q_o = sum_d c[o][d] w_d are rotated,
+// out[f][o][p][k] = sum_q Q_o[k][q] TinvT(q, p) (FP64, the friction law's
+// q_interpolated layout; points k in [(O+1)^2, P) are written as zero).
 template <typename real, int O>
-__global__ __launch_bounds__(kDrThreads) void kDrInterpolate(int64_t nf, const real* __restrict__ V,
-                                                             const real* __restrict__ deriv, int64_t dsize,
-                                                             sdg_dr_layer L, StepCoeffs<O> cf,
-                                                             double* __restrict__ outPlus,
-                                                             double* __restrict__ outMinus) {
-  constexpr int B = numBasis3(O), NP = (O + 1) * (O + 1);
-  __shared__ real sQ[9 * B];
-  __shared__ real sT[81];
-  const int64_t f = blockIdx.x;
-  if (f >= nf) return;
-  const int tid = threadIdx.x;
+__global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, const real* __restrict__ V,
+                                                                const real* __restrict__ deriv, int64_t dsize,
+                                                                sdg_dr_layer L, StepCoeffs<O> cf,
+                                                                double* __restrict__ outPlus,
+                                                                double* __restrict__ outMinus) {
+  constexpr int B = numBasis3(O), NP = (O + 1) * (O + 1), LDV = B + 1;
+  constexpr int DS = [] {
+    int t = 0;
+    for (int d = 0; d < O; ++d) t += 9 * numBasis3(O - d);
+    return t;
+  }();
+  __shared__ real sV[NP * LDV];
+  __shared__ real sD[kDrWaves][DS];
+  __shared__ real sT[kDrWaves][81];
+  __shared__ int sElem[kDrChunk];  // the chunk's element of this variant's side, -1: other variant
+  const int variant = blockIdx.y, vi = variant >> 2, vh = variant & 3;
+  const int side = vh == 0 ? 0 : 1;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const real* Vv = V + static_cast<int64_t>(vi * 4 + vh) * NP * B;
+  const int64_t f0 = static_cast<int64_t>(blockIdx.x) * kDrChunk;
+  if (threadIdx.x < kDrChunk) {
+    const int64_t f = f0 + threadIdx.x;
+    int elem = -1;
+    if (f < nf) {
+      const int faceSide = min(static_cast<int>(L.face_info[f * 3 + side]), 3);
+      const int h = side == 0 ? 0 : min(static_cast<int>(L.face_info[f * 3 + 2]), 3);
+      if (faceSide == vi && h == vh) elem = side == 0 ? L.plus_element[f] : L.minus_element[f];
+    }
+    sElem[threadIdx.x] = elem;
+  }
+  for (int i = threadIdx.x; i < NP * B; i += blockDim.x) sV[(i / B) * LDV + i % B] = Vv[i];
+  __syncthreads();
   const int P = L.points;
-  for (int i = tid; i < 81; i += kDrThreads) sT[i] = static_cast<real>(L.tinv_t[f * 81 + i]);
-  for (int side = 0; side < 2; ++side) {
-    // wave-uniform face data (see kDrNodalFlux)
-    const int64_t e = __builtin_amdgcn_readfirstlane(side == 0 ? L.plus_element[f] : L.minus_element[f]);
-    const int faceSide = __builtin_amdgcn_readfirstlane(static_cast<int>(L.face_info[f * 3 + side]));
-    const int h = side == 0 ? 0 : __builtin_amdgcn_readfirstlane(static_cast<int>(L.face_info[f * 3 + 2]));
-    const real* Vf = V + ((static_cast<int64_t>(min(faceSide, 3)) * 4 + min(h, 3)) * NP) * B;
-    double* out = side == 0 ? outPlus : outMinus;
-    for (int o = 0; o < O; ++o) {
-      __syncthreads();
-      for (int idx = tid; idx < 9 * B; idx += kDrThreads) {
-        const int q = idx / B, l = idx % B;
-        real acc = real(0);
-        int64_t off = e * dsize;
+  double* out = side == 0 ? outPlus : outMinus;
+  for (int j = wave; j < kDrChunk; j += kDrWaves) {
+    const int64_t f = f0 + j;
+    const int elem = sElem[j];
+    if (elem < 0) continue;
+    const int64_t e = elem;
+    waveSync();  // the previous side's reads of sD/sT are done
+    for (int i = lane; i < DS; i += 64) sD[wave][i] = deriv[e * dsize + i];
+    for (int i = lane; i < 81; i += 64) sT[wave][i] = static_cast<real>(L.tinv_t[f * 81 + i]);
+    waveSync();
+    const int k = lane;
+    real w[O][9];
 #pragma unroll
-        for (int d = 0; d < O; ++d) {
-          const int m = numBasis3(O - d);
-          if (l < m) acc += static_cast<real>(cf.c[o][d]) * deriv[off + q * m + l];
-          off += 9 * m;
+    for (int d = 0; d < O; ++d)
+#pragma unroll
+      for (int q = 0; q < 9; ++q) w[d][q] = real(0);
+    if (k < NP) {
+      // derivative order outer, basis index inner: branch-free bodies with
+      // compile-time trip counts that the compiler can unroll and pipeline
+      int off = 0;
+#pragma unroll
+      for (int d = 0; d < O; ++d) {
+        const int m = numBasis3(O - d);
+#pragma unroll 4
+        for (int l = 0; l < m; ++l) {
+          const real v = sV[k * LDV + l];
+#pragma unroll
+          for (int q = 0; q < 9; ++q) w[d][q] += v * sD[wave][off + q * m + l];
         }
-        sQ[idx] = acc;
-      }
-      __syncthreads();
-      const int k = tid;
-      if (k < NP) {
-        real at[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) at[q] = real(0);
-        for (int l = 0; l < B; ++l) {
-          const real v = Vf[k * B + l];
-#pragma unroll
-          for (int q = 0; q < 9; ++q) at[q] += v * sQ[q * B + l];
-        }
-#pragma unroll
-        for (int p = 0; p < 9; ++p) {
-          real acc = real(0);
-#pragma unroll
-          for (int q = 0; q < 9; ++q) acc += at[q] * sT[q + 9 * p];
-          out[((f * O + o) * 9 + p) * P + k] = static_cast<double>(acc);
-        }
-      } else if (k < P) {
-        for (int p = 0; p < 9; ++p) out[((f * O + o) * 9 + p) * P + k] = 0.0;
+        off += 9 * m;
       }
     }
+    // rotation commutes with the time combination: rotate each w_d in place
+#pragma unroll
+    for (int d = 0; d < O; ++d) {
+      real tmp[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) tmp[q] = w[d][q];
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        real acc = real(0);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc += tmp[q] * sT[wave][q + 9 * p];
+        w[d][p] = acc;
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < O; ++o)
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        real acc = real(0);
+#pragma unroll
+        for (int d = 0; d < O; ++d) acc += static_cast<real>(cf.c[o][d]) * w[d][p];
+        if (k < P) out[((f * O + o) * 9 + p) * P + k] = k < NP ? static_cast<double>(acc) : 0.0;
+      }
   }
 }
 
@@ -206,8 +254,9 @@ int interpolateImpl(EngineState& st, const double* coeffs, const void* deriv, co
     for (int d = 0; d < O; ++d) cf.c[o][d] = coeffs[o * O + d];
   int64_t dsize = 0;
   for (int i = 0; i < O; ++i) dsize += 9LL * numBasis3(O - i);
-  kDrInterpolate<real, O><<<static_cast<unsigned>(L->num_faces), kDrThreads, 0, stream>>>(
-      L->num_faces, static_cast<const real*>(st.drMatrices), static_cast<const real*>(deriv), dsize, *L, cf, qp, qm);
+  const dim3 grid(static_cast<unsigned>((L->num_faces + kDrChunk - 1) / kDrChunk), 16);
+  kDrInterpolate<real, O><<<grid, 64 * kDrWaves, 0, stream>>>(L->num_faces, static_cast<const real*>(st.drMatrices),
+                                                              static_cast<const real*>(deriv), dsize, *L, cf, qp, qm);
   SDG_HIP_CHECK(hipGetLastError());
   return SDG_OK;
 }
