@@ -93,7 +93,7 @@ struct Cfg {
   static constexpr int NX = Prec<V>::NX;
   static constexpr bool scaled = Prec<V>::scaled;
   static constexpr int LDX_L = S::KMAX + 8;  // local kernel: padded column stride
-  static constexpr int LDS_L = NX * kCols * LDX_L * 2 + kTileElems * 72 * 4 + kTileElems * 81 * 4 +
+  static constexpr int LDS_L = NX * kCols * LDX_L * 2 + kTileElems * 72 * 4 + 2 * kTileElems * 81 * 4 +
                                (scaled ? S::MT * kCols * 4 : 0);
   static constexpr int WAVES_L = (V != 3 && LDS_L <= 81920) ? 2 : 1;
   static constexpr int LDX_N0 = S::KMAX + 8;
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
   constexpr int TPW = cdiv(S::MTF, MT);
   __shared__ T xs[NX * kCols * LDX];
   __shared__ float sStar[kTileElems * 72];
-  __shared__ float sAp[kTileElems * 81];
+  __shared__ float sAp[2][kTileElems * 81];  // AplusT of faces f and f + 1
   __shared__ float sMax[scaled ? MT * kCols : 1];
 
   const int lane = threadIdx.x & 63;
@@ -527,19 +527,28 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     }
   }
 
-  // W = T AplusT_f in place of T, face by face (AplusT staged in LDS)
+  // W = T AplusT_f in place of T, face by face.  AplusT is staged in two LDS
+  // buffers: face f + 1 is stored while face f is applied, one barrier per face
+  // (the buffer written in iteration f was last read in iteration f - 1).
+#pragma unroll
+  for (int i = 0; i < APT; ++i) {
+    const int idx = threadIdx.x + i * S::THREADS;
+    if (idx < kTileElems * 81) sAp[0][idx] = apv[i];
+  }
+  loadAplus(1);
+  __syncthreads();
 #pragma unroll 1
   for (int f = 0; f < 4; ++f) {
-    __syncthreads();
+    if (f < 3) {
 #pragma unroll
-    for (int i = 0; i < APT; ++i) {
-      const int idx = threadIdx.x + i * S::THREADS;
-      if (idx < kTileElems * 81) sAp[idx] = apv[i];
+      for (int i = 0; i < APT; ++i) {
+        const int idx = threadIdx.x + i * S::THREADS;
+        if (idx < kTileElems * 81) sAp[(f + 1) & 1][idx] = apv[i];
+      }
+      if (f < 2) loadAplus(f + 2);
     }
-    if (f < 3) loadAplus(f + 1);
-    __syncthreads();
     const bool dr = valid && faceTypes != nullptr && faceTypes[elem * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE;
-    const float* A = sAp + e * 81;
+    const float* A = sAp[f & 1] + e * 81;
 #pragma unroll
     for (int ti = 0; ti < TPW; ++ti) {
       const int t = w + ti * MT;
@@ -560,6 +569,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
         }
       }
     }
+    __syncthreads();
   }
   // volume: Q += sum_j kDivM_j I star_j (X still holds I; computed after W so
   // that its result is not live across the face products)
