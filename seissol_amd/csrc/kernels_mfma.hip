@@ -98,7 +98,7 @@ struct Cfg {
   static constexpr int WAVES_L = (V != 3 && LDS_L <= 81920) ? 2 : 1;
   static constexpr int LDX_N0 = S::KMAX + 8;
   static constexpr int lds_n(int ldx) {
-    return NX * kCols * ldx * 2 + kTileElems * 9 * S::B2P * 4 + kTileElems * 81 * 4 + 3 * S::B2 * S::B2 * 4 +
+    return NX * kCols * ldx * 2 + kTileElems * 9 * S::B2P * 4 + kTileElems * 81 * 4 + 3 * S::B2 * O * 8 +
            (scaled ? kCols * 4 : 0);
   }
   // neighbour kernel: drop the padding when that makes two workgroups fit
@@ -630,7 +630,8 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   __shared__ T xs[NX * kCols * LDX];
   __shared__ __attribute__((aligned(16))) float sTr[kTileElems * 9 * S::B2P];
   __shared__ float sAm[kTileElems * 81];
-  __shared__ float sFP[3 * B2 * B2];
+  __shared__ float sFP[3 * B2 * O];   // fP_h rows: O (value, column) entries each
+  __shared__ int sFPCol[3 * B2 * O];
   __shared__ unsigned sMaxU[scaled ? kCols : 1];
   __shared__ int sLink[kTileElems * 4];  // trace index nb * 4 + side of (element, face), -1: none
   __shared__ int sRel[kTileElems * 4];   // orientation h of (element, face)
@@ -648,7 +649,10 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   float* const tileDofs = dofs + E0 * 9 * B;
   const int laneQ = e * 9 * B + 16 * w + 4 * g;
 
-  for (int i = threadIdx.x; i < 3 * B2 * B2; i += blockDim.x) sFP[i] = tables[S::tFP + i];
+  for (int i = threadIdx.x; i < 3 * B2 * O; i += blockDim.x) {
+    sFP[i] = tables[S::tFP + i];
+    sFPCol[i] = static_cast<int>(tables[S::tFPCol + i]);
+  }
   // neighbour links of the four faces, read once
   if (threadIdx.x < kTileElems * 4) {
     const int s = threadIdx.x >> 2, f = threadIdx.x & 3;
@@ -721,10 +725,13 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
         float u[9];
 #pragma unroll
         for (int qq = 0; qq < 9; ++qq) u[qq] = 0.0f;
-        const float* fp = sFP + (h * B2 + row) * B2;
+        const float* fp = sFP + (h * B2 + row) * O;
+        const int* fc = sFPCol + (h * B2 + row) * O;
         const float* tr = sTr + s * TR;
-        for (int mm = 0; mm < B2; ++mm) {
-          const float c = fp[mm];
+#pragma unroll
+        for (int kk = 0; kk < O; ++kk) {
+          const float c = fp[kk];
+          const int mm = fc[kk];
 #pragma unroll
           for (int qq = 0; qq < 9; ++qq) u[qq] += c * tr[qq * S::B2P + mm];
         }

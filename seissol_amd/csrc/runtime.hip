@@ -155,7 +155,25 @@ int uploadMfmaData(EngineState& s, const double* gm) {
   add([&](int r, int c) { return c % B2P < B2 ? gm[g.rDivM + ((c / B2P) * B + r) * B2 + c % B2P] : 0.0; }, B, S::FR,
       S::MT, S::KSF, S::rdm, 7);
   for (int m = 0; m < B2; ++m) tables[S::tInvM2 + m] = static_cast<float>(1.0 / gm[g.mass2 + m]);
-  for (int i = 0; i < 3 * B2 * B2; ++i) tables[S::tFP + i] = static_cast<float>(gm[g.fP + i]);
+  // fP_h rows as (value, column) lists of O entries, padded with zeros
+  for (int h = 0; h < 3; ++h) {
+    double mx = 0.0;
+    for (int i = 0; i < B2 * B2; ++i) mx = std::max(mx, std::fabs(gm[g.fP + h * B2 * B2 + i]));
+    for (int r = 0; r < B2; ++r) {
+      int cnt = 0;
+      for (int c = 0; c < B2; ++c) {
+        const double v = gm[g.fP + (h * B2 + r) * B2 + c];
+        if (std::fabs(v) <= 1e-13 * mx) continue;
+        if (cnt == O) {
+          setError("face permutation matrix with more than `order` non-zeros in a row");
+          return SDG_ERR_INVALID;
+        }
+        tables[S::tFP + (h * B2 + r) * O + cnt] = static_cast<float>(v);
+        tables[S::tFPCol + (h * B2 + r) * O + cnt] = static_cast<float>(c);
+        ++cnt;
+      }
+    }
+  }
   if (s.mfmaFrags == nullptr) SDG_HIP_CHECK(hipMalloc(&s.mfmaFrags, frags.size() * sizeof(uint16_t)));
   if (s.mfmaTables == nullptr) SDG_HIP_CHECK(hipMalloc(&s.mfmaTables, tables.size() * sizeof(float)));
   SDG_HIP_CHECK(hipMemcpy(s.mfmaFrags, frags.data(), frags.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
