@@ -307,12 +307,14 @@ __device__ inline void updateRows(float* q, int row0, const floatx4 (&acc)[9], f
 // loaded inside the pass (L2-resident), which keeps them out of the live set.
 template <int V, int KSMAX, int LDX>
 __device__ inline void gemmStars(const uint16_t* const (&a)[3], int ks, const typename Prec<V>::T* xs, int lane,
-                                 const float* sS, const float (&aUn)[3], const float (&colUn)[9], float (&y)[9][4]) {
+                                 const float* sS, const float (&aUn)[3], const float (&colUn)[9], float (&y)[9][4],
+                                 int passMask = 7) {
   using P = Prec<V>;
   constexpr int NX = P::NX;
   const int g = lane >> 4, e = lane & 15;
 #pragma unroll
   for (int pass = 0; pass < 3; ++pass) {
+    if (((passMask >> pass) & 1) == 0) continue;  // wave-uniform
     const int q0 = pass == 0 ? 6 : (pass == 1 ? 0 : 3);
     floatx4 acc[3][3];
 #pragma unroll
@@ -418,21 +420,62 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
   __syncthreads();
   const float* sS = sStar + e * 72;
 
-  // Cauchy-Kowalevski recursion
+  // Cauchy-Kowalevski recursion.  Wave w owns m-tile w of dQ_d.  When dQ_d
+  // has fewer m-tiles than there are waves, idle waves take over velocity
+  // passes (input groups {0,1,2}, {3,4,5}) of the owned tiles and hand their
+  // partial sums back through LDS (the AplusT staging area, unused until the
+  // face phase): mode 2 = passes 1 and 2 of a single tile, mode 1 = pass 2 of
+  // every tile, 0 = none.
+  constexpr int kSlotFloats = 3 * 16 * kTileElems;  // 3 velocities x 16 rows x 16 elements
+  constexpr int kSlots = (2 * kTileElems * 81) / kSlotFloats;
+  float* const stage = &sAp[0][0];
 #pragma unroll 1
   for (int d = 1; d < O; ++d) {
     const int m = basis3(O - d), nIn = basis3(O - d + 1);
     const int mtiles = cdiv(m, 16), ks = cdiv(nIn, 32), kpNext = cdiv(m, 32) * 32;
+    const int helpers = MT - mtiles;
+    const int mode = (mtiles == 1 && helpers >= 2 && kSlots >= 2) ? 2 : ((helpers >= mtiles && mtiles <= kSlots) ? 1 : 0);
+    int tile = -1, mask = 0, slot = -1;
+    if (w < mtiles) {
+      tile = w;
+      mask = mode == 2 ? 1 : (mode == 1 ? 3 : 7);
+    } else if (mode == 2 && w - mtiles < 2) {
+      tile = 0;
+      mask = 2 << (w - mtiles);
+      slot = w - mtiles;
+    } else if (mode == 1 && w - mtiles < mtiles) {
+      tile = w - mtiles;
+      mask = 4;
+      slot = tile;
+    }
     float dq[1][9][4];
 #pragma unroll
     for (int p = 0; p < 9; ++p)
 #pragma unroll
       for (int j = 0; j < 4; ++j) dq[0][p][j] = 0.0f;
+    if (tile >= 0) {
+      const uint16_t* a[3] = {frag + S::ck + (0LL * MT + tile) * S::KSB * kFragPair,
+                              frag + S::ck + (1LL * MT + tile) * S::KSB * kFragPair,
+                              frag + S::ck + (2LL * MT + tile) * S::KSB * kFragPair};
+      gemmStars<V, S::KSB, LDX>(a, ks, xs, lane, sS, aUnCk, un, dq[0], mask);
+    }
+    if (mode != 0) {
+      if (slot >= 0)
+#pragma unroll
+        for (int p = 6; p < 9; ++p)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) stage[slot * kSlotFloats + (p - 6) * 256 + (4 * g + j) * 16 + e] = dq[0][p][j];
+      __syncthreads();
+      if (w < mtiles) {
+        const int s0 = mode == 2 ? 0 : w, ns = mode == 2 ? 2 : 1;
+        for (int sl = s0; sl < s0 + ns; ++sl)
+#pragma unroll
+          for (int p = 6; p < 9; ++p)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) dq[0][p][j] += stage[sl * kSlotFloats + (p - 6) * 256 + (4 * g + j) * 16 + e];
+      }
+    }
     if (w < mtiles) {
-      const uint16_t* a[3] = {frag + S::ck + (0LL * MT + w) * S::KSB * kFragPair,
-                              frag + S::ck + (1LL * MT + w) * S::KSB * kFragPair,
-                              frag + S::ck + (2LL * MT + w) * S::KSB * kFragPair};
-      gemmStars<V, S::KSB, LDX>(a, ks, xs, lane, sS, aUnCk, un, dq[0]);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         if (16 * w + 4 * g + j >= m)
