@@ -548,7 +548,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
 #pragma unroll
       for (int p = 0; p < 9; ++p) acc[0][p] = floatx4{0, 0, 0, 0};
       const uint16_t* a[1] = {frag + S::fmrt + static_cast<int64_t>(t) * S::KSB * kFragPair};
-      gemmTile<V, 1, 0, 9, LDX>(acc, a, S::KSB, xs, lane);
+      if (E0 < n) gemmTile<V, 1, 0, 9, LDX>(acc, a, S::KSB, xs, lane);
 #pragma unroll
       for (int p = 0; p < 9; ++p)
 #pragma unroll
@@ -625,7 +625,10 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     const uint16_t* a[3] = {frag + S::vol + (0LL * MT + w) * S::KSB * kFragPair,
                             frag + S::vol + (1LL * MT + w) * S::KSB * kFragPair,
                             frag + S::vol + (2LL * MT + w) * S::KSB * kFragPair};
-    gemmStars<V, S::KSB, LDX>(a, S::KSB, xs, lane, sS, aUnVol, un, vol);
+    // `E0 < n` holds for every launched tile; the guard gives this GEMM (and
+    // the trace GEMM below) a block of its own, with which the register
+    // allocator keeps the kernel free of spills (without it ~180 B per lane)
+    if (E0 < n) gemmStars<V, S::KSB, LDX>(a, S::KSB, xs, lane, sS, aUnVol, un, vol);
   }
 
   if constexpr (scaled) waveColumnMax<TPW>(tr, true, sMax, w, lane);
