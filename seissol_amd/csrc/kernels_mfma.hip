@@ -93,7 +93,7 @@ struct Cfg {
   static constexpr int NX = Prec<V>::NX;
   static constexpr bool scaled = Prec<V>::scaled;
   static constexpr int LDX_L = S::KMAX + 8;  // local kernel: padded column stride
-  static constexpr int LDS_L = NX * kCols * LDX_L * 2 + kTileElems * 72 * 4 + 2 * kTileElems * 81 * 4 +
+  static constexpr int LDS_L = NX * kCols * LDX_L * 2 + kTileElems * 72 * 4 + 9 * 16 * kTileElems * 4 +
                                (scaled ? S::MT * kCols * 4 : 0);
   static constexpr int WAVES_L = (V != 3 && LDS_L <= 81920) ? 2 : 1;
   static constexpr int LDX_N0 = S::KMAX + 8;
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
   constexpr int TPW = cdiv(S::MTF, MT);
   __shared__ T xs[NX * kCols * LDX];
   __shared__ float sStar[kTileElems * 72];
-  __shared__ float sAp[2][kTileElems * 81];  // AplusT of faces f and f + 1
+  __shared__ float sStage[3 * 3 * 16 * kTileElems];  // CK partial sums handed between waves (3 slots)
   __shared__ float sMax[scaled ? MT * kCols : 1];
 
   const int lane = threadIdx.x & 63;
@@ -423,12 +423,11 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
   // Cauchy-Kowalevski recursion.  Wave w owns m-tile w of dQ_d.  When dQ_d
   // has fewer m-tiles than there are waves, idle waves take over velocity
   // passes (input groups {0,1,2}, {3,4,5}) of the owned tiles and hand their
-  // partial sums back through LDS (the AplusT staging area, unused until the
-  // face phase): mode 2 = passes 1 and 2 of a single tile, mode 1 = pass 2 of
+  // partial sums back through LDS: mode 2 = passes 1 and 2 of a single tile, mode 1 = pass 2 of
   // every tile, 0 = none.
   constexpr int kSlotFloats = 3 * 16 * kTileElems;  // 3 velocities x 16 rows x 16 elements
-  constexpr int kSlots = (2 * kTileElems * 81) / kSlotFloats;
-  float* const stage = &sAp[0][0];
+  constexpr int kSlots = 3;
+  float* const stage = sStage;
 #pragma unroll 1
   for (int d = 1; d < O; ++d) {
     const int m = basis3(O - d), nIn = basis3(O - d + 1);
@@ -521,19 +520,6 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     for (int p = 0; p < 9; ++p) storeZero4<V, LDX>(xs, p * 16 + e, 16 * t + 4 * g);
   __syncthreads();
 
-  // AplusT of face 0 in flight during the trace GEMM; face f + 1 while face f is applied
-  constexpr int APT = cdiv(kTileElems * 81, S::THREADS);
-  float apv[APT];
-  auto loadAplus = [&](int f) {
-#pragma unroll
-    for (int i = 0; i < APT; ++i) {
-      const int idx = threadIdx.x + i * S::THREADS;
-      apv[i] = (idx < kTileElems * 81 && E0 + idx / 81 < n) ? aplus[E0 * 324 + (idx / 81) * 324 + f * 81 + idx % 81]
-                                                             : 0.0f;
-    }
-  };
-  loadAplus(0);
-
   // face traces T = fMrT_f I (rows f * B2 + m)
   float tr[TPW][9][4];
 #pragma unroll
@@ -570,52 +556,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     }
   }
 
-  // W = T AplusT_f in place of T, face by face.  AplusT is staged in two LDS
-  // buffers: face f + 1 is stored while face f is applied, one barrier per face
-  // (the buffer written in iteration f was last read in iteration f - 1).
-#pragma unroll
-  for (int i = 0; i < APT; ++i) {
-    const int idx = threadIdx.x + i * S::THREADS;
-    if (idx < kTileElems * 81) sAp[0][idx] = apv[i];
-  }
-  loadAplus(1);
-  __syncthreads();
-#pragma unroll 1
-  for (int f = 0; f < 4; ++f) {
-    if (f < 3) {
-#pragma unroll
-      for (int i = 0; i < APT; ++i) {
-        const int idx = threadIdx.x + i * S::THREADS;
-        if (idx < kTileElems * 81) sAp[(f + 1) & 1][idx] = apv[i];
-      }
-      if (f < 2) loadAplus(f + 2);
-    }
-    const bool dr = valid && faceTypes != nullptr && faceTypes[elem * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE;
-    const float* A = sAp[f & 1] + e * 81;
-#pragma unroll
-    for (int ti = 0; ti < TPW; ++ti) {
-      const int t = w + ti * MT;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int r = 16 * t + 4 * g + j;
-        if (t < S::MTF && r < FR && r / S::B2P == f) {
-          float wrow[9];
-#pragma unroll
-          for (int p = 0; p < 9; ++p) {
-            float acc = 0.0f;
-#pragma unroll
-            for (int qq = 0; qq < 9; ++qq) acc += tr[ti][qq][j] * A[qq + 9 * p];
-            wrow[p] = dr ? 0.0f : acc;
-          }
-#pragma unroll
-          for (int p = 0; p < 9; ++p) tr[ti][p][j] = wrow[p];
-        }
-      }
-    }
-    __syncthreads();
-  }
-  // volume: Q += sum_j kDivM_j I star_j (X still holds I; computed after W so
-  // that its result is not live across the face products)
+  // volume: Q += sum_j kDivM_j I star_j (X holds I)
   float vol[9][4];
 #pragma unroll
   for (int p = 0; p < 9; ++p)
@@ -626,9 +567,70 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
                             frag + S::vol + (1LL * MT + w) * S::KSB * kFragPair,
                             frag + S::vol + (2LL * MT + w) * S::KSB * kFragPair};
     // `E0 < n` holds for every launched tile; the guard gives this GEMM (and
-    // the trace GEMM below) a block of its own, with which the register
-    // allocator keeps the kernel free of spills (without it ~180 B per lane)
+    // the trace GEMM above) a block of its own, with which the register
+    // allocator keeps the kernel free of spills
     if (E0 < n) gemmStars<V, S::KSB, LDX>(a, S::KSB, xs, lane, sS, aUnVol, un, vol);
+  }
+
+  // Once every wave has finished the volume GEMM, the X area is free until W
+  // is stored into it: it takes the four faces' AplusT and (when it fits) the
+  // volume term, so that W = T AplusT_f is formed by every lane at once with
+  // its own face's matrix (a lane's four rows of a tile lie in one face).
+  constexpr int kXBytes = NX * kCols * LDX * 2;
+  constexpr bool kParkVol = (kTileElems * 324 + kTileElems * 9 * B) * 4 <= kXBytes;
+  static_assert(kTileElems * 324 * 4 <= kXBytes, "the X area holds the tile's AplusT");
+  float* const sA4 = reinterpret_cast<float*>(xs);
+  float* const sVol = sA4 + kTileElems * 324;  // [element][quantity][B]
+  constexpr int APT = cdiv(kTileElems * 324, S::THREADS);  // AplusT of the four faces, [element][face][81]
+  float apv[APT];
+#pragma unroll
+  for (int i = 0; i < APT; ++i) {
+    const int idx = threadIdx.x + i * S::THREADS;
+    apv[i] = (idx < kTileElems * 324 && E0 + idx / 324 < n) ? aplus[E0 * 324 + idx] : 0.0f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < APT; ++i) {
+    const int idx = threadIdx.x + i * S::THREADS;
+    if (idx < kTileElems * 324) sA4[idx] = apv[i];
+  }
+  if constexpr (kParkVol) {
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (16 * w + 4 * g + j < B) sVol[(e * 9 + p) * B + 16 * w + 4 * g + j] = vol[p][j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int ti = 0; ti < TPW; ++ti) {
+    const int t = w + ti * MT;
+    const int r0 = 16 * t + 4 * g;
+    if (t < S::MTF && r0 < FR) {
+      const int f = r0 / S::B2P;
+      const bool dr = valid && faceTypes != nullptr && faceTypes[elem * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE;
+      const float* A = sA4 + e * 324 + f * 81;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        float wrow[9];
+#pragma unroll
+        for (int p = 0; p < 9; ++p) {
+          float acc = 0.0f;
+#pragma unroll
+          for (int qq = 0; qq < 9; ++qq) acc += tr[ti][qq][j] * A[qq + 9 * p];
+          wrow[p] = dr ? 0.0f : acc;
+        }
+#pragma unroll
+        for (int p = 0; p < 9; ++p) tr[ti][p][j] = wrow[p];
+      }
+    }
+  }
+  if constexpr (kParkVol) {
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        vol[p][j] = 16 * w + 4 * g + j < B ? sVol[(e * 9 + p) * B + 16 * w + 4 * g + j] : 0.0f;
   }
 
   if constexpr (scaled) waveColumnMax<TPW>(tr, true, sMax, w, lane);
