@@ -104,7 +104,7 @@ struct Cfg {
   // neighbour kernel: drop the padding when that makes two workgroups fit
   static constexpr int LDX_N = (V != 3 && lds_n(LDX_N0) > 81920 && lds_n(S::KMAX) <= 81920) ? S::KMAX : LDX_N0;
   static constexpr int WAVES_N = (V != 3 && lds_n(LDX_N) <= 81920) ? 2 : 1;
-  static constexpr int IPT = cdiv(kTileElems * S::B2, S::THREADS);  // face items per thread
+  static constexpr int IPT = cdiv(kTileElems * S::B2P / 2, S::THREADS);  // face row pairs per thread
 };
 
 __device__ inline floatx4 mma(bf16x8 a, bf16x8 b, floatx4 c) {
@@ -163,6 +163,18 @@ __device__ inline void storeX1(typename Prec<V>::T* xs, int col, int k, float v)
   split<V>(v, parts);
 #pragma unroll
   for (int i = 0; i < Prec<V>::NX; ++i) xs[i * kCols * LDX + col * LDX + k] = parts[i];
+}
+
+// two consecutive rows k0, k0 + 1 (k0 even) of one column: one 32-bit store per plane
+template <int V, int LDX>
+__device__ inline void storeX2(typename Prec<V>::T* xs, int col, int k0, float v0, float v1) {
+  using P = Prec<V>;
+  typedef typename P::T T2 __attribute__((ext_vector_type(2)));
+  typename P::T a[P::NX], b[P::NX];
+  split<V>(v0, a);
+  split<V>(v1, b);
+#pragma unroll
+  for (int i = 0; i < P::NX; ++i) *reinterpret_cast<T2*>(xs + i * kCols * LDX + col * LDX + k0) = T2{a[i], b[i]};
 }
 
 template <int V, int LDX>
@@ -674,7 +686,8 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   using T = typename Prec<V>::T;
   constexpr int B = S::B, B2 = S::B2, LDX = C::LDX_N, FR = S::FR, NX = C::NX, IPT = C::IPT;
   constexpr bool scaled = C::scaled;
-  constexpr int ITEMS = kTileElems * B2;
+  constexpr int HALF = S::B2P / 2;              // row pairs of one (padded) face
+  constexpr int ITEMS = kTileElems * HALF;
   __shared__ T xs[NX * kCols * LDX];
   __shared__ __attribute__((aligned(16))) float sTr[kTileElems * 9 * S::B2P];
   __shared__ float sAm[kTileElems * 81];
@@ -716,11 +729,11 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   if constexpr (scaled)
     for (int i = threadIdx.x; i < kCols; i += blockDim.x) sMaxU[i] = 0u;
   // zero padding rows of the face operand
-  for (int i = threadIdx.x; i < kCols * S::KPF; i += blockDim.x) {
-    const int col = i / S::KPF, k = i % S::KPF;
-    if (k >= FR || k % S::B2P >= B2)
+  // rows beyond the four faces (the face rows' padding is written as zeros by the items)
+  for (int i = threadIdx.x; i < kCols * (S::KPF - FR); i += blockDim.x) {
+    const int col = i / (S::KPF - FR), k = FR + i % (S::KPF - FR);
 #pragma unroll
-      for (int pl = 0; pl < NX; ++pl) xs[pl * kCols * LDX + col * LDX + k] = static_cast<T>(0.0f);
+    for (int pl = 0; pl < NX; ++pl) xs[pl * kCols * LDX + col * LDX + k] = static_cast<T>(0.0f);
   }
   __syncthreads();
   // the next face's traces and AminusT are loaded into registers while the
@@ -747,7 +760,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   };
   gather(0);
   // the fp16 form keeps the face operand in registers until its column scales are known
-  float wv[scaled ? 4 : 1][scaled ? IPT : 1][9];
+  float wv[scaled ? 4 : 1][scaled ? IPT : 1][2][9];
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
     __syncthreads();
@@ -767,34 +780,46 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
     for (int it = 0; it < IPT; ++it) {
       const int item = threadIdx.x + it * S::THREADS;
       if (item < ITEMS) {
-        const int s = item / B2, row = item % B2;
+        // rows 2 rp and 2 rp + 1 of face f of element s (rows >= B2 are padding: zero)
+        const int s = item / HALF, r0 = 2 * (item % HALF);
         const bool regular = sLink[s * 4 + f] >= 0;
         const int h = sRel[s * 4 + f];
-        float u[9];
-#pragma unroll
-        for (int qq = 0; qq < 9; ++qq) u[qq] = 0.0f;
-        const float* fp = sFP + (h * B2 + row) * O;
-        const int* fc = sFPCol + (h * B2 + row) * O;
         const float* tr = sTr + s * TR;
+        float u[2][9];
 #pragma unroll
-        for (int kk = 0; kk < O; ++kk) {
-          const float c = fp[kk];
-          const int mm = fc[kk];
+        for (int ri = 0; ri < 2; ++ri) {
 #pragma unroll
-          for (int qq = 0; qq < 9; ++qq) u[qq] += c * tr[qq * S::B2P + mm];
+          for (int qq = 0; qq < 9; ++qq) u[ri][qq] = 0.0f;
+          const int row = r0 + ri;
+          if (row < B2) {
+            const float* fp = sFP + (h * B2 + row) * O;
+            const int* fc = sFPCol + (h * B2 + row) * O;
+#pragma unroll
+            for (int kk = 0; kk < O; ++kk) {
+              const float c = fp[kk];
+              const int mm = fc[kk];
+#pragma unroll
+              for (int qq = 0; qq < 9; ++qq) u[ri][qq] += c * tr[qq * S::B2P + mm];
+            }
+          }
         }
         const float* A = sAm + s * 81;
 #pragma unroll
         for (int p = 0; p < 9; ++p) {
-          float acc = 0.0f;
+          float acc0 = 0.0f, acc1 = 0.0f;
           if (regular)
 #pragma unroll
-            for (int qq = 0; qq < 9; ++qq) acc += u[qq] * A[qq + 9 * p];
+            for (int qq = 0; qq < 9; ++qq) {
+              const float a = A[qq + 9 * p];
+              acc0 += u[0][qq] * a;
+              acc1 += u[1][qq] * a;
+            }
           if constexpr (scaled) {
-            wv[f][it][p] = acc;
-            atomicMax(&sMaxU[p * 16 + s], __float_as_uint(fabsf(acc)));
+            wv[f][it][0][p] = acc0;
+            wv[f][it][1][p] = acc1;
+            atomicMax(&sMaxU[p * 16 + s], __float_as_uint(fmaxf(fabsf(acc0), fabsf(acc1))));
           } else {
-            storeX1<V, LDX>(xs, p * 16 + s, f * S::B2P + row, acc);
+            storeX2<V, LDX>(xs, p * 16 + s, f * S::B2P + r0, acc0, acc1);
           }
         }
       }
@@ -811,12 +836,12 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
       for (int it = 0; it < IPT; ++it) {
         const int item = threadIdx.x + it * S::THREADS;
         if (item < ITEMS) {
-          const int s = item / B2, row = item % B2;
+          const int s = item / HALF, r0 = 2 * (item % HALF);
 #pragma unroll
           for (int p = 0; p < 9; ++p) {
             float sc, inv;
             columnScale(__uint_as_float(sMaxU[p * 16 + s]), sc, inv);
-            storeX1<V, LDX>(xs, p * 16 + s, f * S::B2P + row, wv[f][it][p] * sc);
+            storeX2<V, LDX>(xs, p * 16 + s, f * S::B2P + r0, wv[f][it][0][p] * sc, wv[f][it][1][p] * sc);
           }
         }
       }
