@@ -36,6 +36,11 @@ bool drLayerOk(const sdg_dr_layer* L, int order, const char* who) {
     setError(std::string(who) + ": invalid DR layer");
     return false;
   }
+  if (stroudPoints(order) > 64) {
+    setError(std::string(who) + ": dynamic rupture supports orders up to 7 (the (order+1)^2 points of a face "
+             "are the lanes of one wave)");
+    return false;
+  }
   if (L->num_faces == 0) return true;
   if (L->points < stroudPoints(order) || L->points > 64 || L->plus_element == nullptr ||
       L->minus_element == nullptr || L->face_info == nullptr) {

@@ -17,7 +17,7 @@ int hipFail(hipError_t err, const char* what);
     if (err_ != hipSuccess) return ::sdg::hipFail(err_, #call); \
   } while (0)
 
-constexpr int kMaxOrder = 7;
+constexpr int kMaxOrder = 8;
 constexpr int kMinOrder = 2;
 
 // Offsets (in reals) of the device copies of the global matrices.  All

@@ -670,7 +670,7 @@ int dispatch(const EngineState& st, Args&&... args) {
 #define SDG_CASE(o)                                                            \
   case o:                                                                      \
     return dbl ? F<double, o>::run(st, args...) : F<float, o>::run(st, args...);
-    SDG_CASE(2) SDG_CASE(3) SDG_CASE(4) SDG_CASE(5) SDG_CASE(6) SDG_CASE(7)
+    SDG_CASE(2) SDG_CASE(3) SDG_CASE(4) SDG_CASE(5) SDG_CASE(6) SDG_CASE(7) SDG_CASE(8)
 #undef SDG_CASE
     default:
       setError("unsupported order");

@@ -117,7 +117,7 @@ using namespace sdg;
 extern "C" int sdg_performance_estimate(int order, int arithmetic, int kernel_id, sdg_work_estimate* out) {
   if (out == nullptr || order < kMinOrder || order > kMaxOrder || arithmetic < SDG_FP64 ||
       arithmetic > SDG_F16X3 || kernel_id < 0 || kernel_id > 7 ||
-      (kernel_id >= 5 && arithmetic != SDG_FP64 && arithmetic != SDG_FP32)) {
+      (kernel_id >= 5 && (order > 7 || (arithmetic != SDG_FP64 && arithmetic != SDG_FP32)))) {
     setError("sdg_performance_estimate: invalid order, arithmetic, kernel or NULL output");
     return SDG_ERR_INVALID;
   }

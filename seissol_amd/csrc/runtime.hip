@@ -360,7 +360,7 @@ int sdg_engine_create(sdg_engine** engine, int order, int arithmetic) {
   if (engine == nullptr) { setError("engine is NULL"); return SDG_ERR_INVALID; }
   *engine = nullptr;
   if (order < kMinOrder || order > kMaxOrder) {
-    setError("sdg_engine_create: order must be in [2, 7]");
+    setError("sdg_engine_create: order must be in [2, 8]");
     return SDG_ERR_UNSUPPORTED;
   }
   if (arithmetic < SDG_FP64 || arithmetic > SDG_F16X3) {

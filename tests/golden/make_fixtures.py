@@ -9,7 +9,7 @@ reference at run time.  Only numeric data is extracted:
 * ``godunov_elastic.json`` -- the elastic Godunov-state golden matrices and
   materials of tests/Model/Values.h:24-445, 475-477;
 * ``global_matrix_checksums.json`` -- per-matrix sums / norms of
-  codegen/matrices/matrices_{4,10,20,35,56,84}.xml;
+  codegen/matrices/matrices_{4,10,20,35,56,84,120}.xml;
 * ``dr_stroud.json`` -- quadrature points, weights, the resampling matrix and
   the face evaluation matrices V3mTo2n / V3mTo2nTWDivM of
   codegen/matrices/dr_stroud_matrices_{4,5,6}.json (checksums + order-4 data);
@@ -134,7 +134,7 @@ def checksums(a: np.ndarray) -> dict:
 
 def global_matrix_checksums(ref: Path) -> dict:
     out = {}
-    for order in range(2, 8):
+    for order in range(2, 9):
         B = order * (order + 1) * (order + 2) // 6
         mats = parse_xml(ref / f"codegen/matrices/matrices_{B}.xml")
         out[str(order)] = {name: checksums(a) for name, a in mats.items()

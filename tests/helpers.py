@@ -96,7 +96,18 @@ class PolyField:
         px = np.stack([s[..., 0] ** k for k in range(O)], -1)
         py = np.stack([s[..., 1] ** k for k in range(O)], -1)
         pz = np.stack([s[..., 2] ** k for k in range(O)], -1)
-        return np.einsum("...a,...b,...c,pabc->...p", px, py, pz, self.c)
+        # sum_abc x^a y^b z^c c[p,a,b,c]: the z sum as one matrix product per
+        # chunk of points, then a batched product with the x^a y^b weights
+        m = px.reshape(-1, O).shape[0]
+        px, py, pz = px.reshape(m, O), py.reshape(m, O), pz.reshape(m, O)
+        cz = self.c.transpose(3, 0, 1, 2).reshape(O, 9 * O * O)
+        out = np.empty((m, 9))
+        for i in range(0, m, 32768):
+            j = min(m, i + 32768)
+            t = (pz[i:j] @ cz).reshape(j - i, 9, O * O)
+            w = (px[i:j, :, None] * py[i:j, None, :]).reshape(j - i, O * O, 1)
+            out[i:j] = (t @ w)[..., 0]
+        return out.reshape(s.shape[:-1] + (9,))
 
 
 def evolve(field: PolyField, A, t: float, integrate: bool = False) -> PolyField:
@@ -118,7 +129,7 @@ def project(order: int, vertices: np.ndarray, field: PolyField, nq: int | None =
     J = np.stack([vertices[:, 1] - v0, vertices[:, 2] - v0, vertices[:, 3] - v0], -1)  # [n][3][3]
     x = v0[:, None, :] + np.einsum("nij,qj->nqi", J, pts)
     f = field.evaluate(x)  # [n][q][9]
-    return np.einsum("qk,q,nqp->npk", phi, w, f) / mass[None, None, :]
+    return np.einsum("qk,nqp->npk", phi * w[:, None], f, optimize=True) / mass[None, None, :]
 
 
 def interior_elements(setup, margin: int = 1) -> np.ndarray:

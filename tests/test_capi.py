@@ -37,6 +37,9 @@ def test_errors_are_reported():
     rc = lib.sdg_engine_create(ctypes.byref(eng), 12, _capi.SDG_FP64)
     assert rc != 0
     assert b"order" in lib.sdg_last_error()
+    # orders 2..8 as the reference's matrices_{4..120}.xml; 9 is rejected before any device call
+    assert lib.sdg_engine_create(ctypes.byref(eng), 9, _capi.SDG_FP64) != 0
+    assert b"[2, 8]" in lib.sdg_last_error()
     assert lib.sdg_proxy_kernel_id(b"all") == 0
     assert lib.sdg_proxy_kernel_id(b"nonsense") < 0
 
@@ -66,6 +69,12 @@ def test_work_estimates_match_the_reference_sparsity():
     # split-bf16 forms execute 3 or 6 products per dense contraction
     assert _estimate(6, _capi.SDG_BF16X6, 0).hardware_flop == 6 * _estimate(6, _capi.SDG_FP32, 0).hardware_flop
     assert _capi.lib().sdg_performance_estimate(6, 0, 8, ctypes.byref(_capi.WorkEstimate())) != 0
+    # ORDER 8 element kernels are estimated; its DR kernels are not supported
+    B8 = 120
+    assert _estimate(8, _capi.SDG_FP64, 0).bytes == (6 * 9 * B8 + 72 + 8 * 81) * 8
+    assert _estimate(8, _capi.SDG_FP64, 4).nonzero_flop > _estimate(7, _capi.SDG_FP64, 4).nonzero_flop
+    assert _capi.lib().sdg_performance_estimate(8, 0, 5, ctypes.byref(_capi.WorkEstimate())) != 0
+    assert _capi.lib().sdg_performance_estimate(9, 0, 0, ctypes.byref(_capi.WorkEstimate())) != 0
 
 
 def test_dr_work_estimates_known_answers():

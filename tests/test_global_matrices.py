@@ -17,7 +17,7 @@ def checksums(a):
             "max_abs": np.abs(a).max(), "weighted": (a * np.arange(1, a.size + 1).reshape(a.shape)).sum()}
 
 
-@pytest.mark.parametrize("order", [2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("order", [2, 3, 4, 5, 6, 7, 8])
 def test_matrix_checksums_match_reference_data(order):
     gold = json.loads((GOLDEN / "global_matrix_checksums.json").read_text())[str(order)]
     gm = global_matrices(order)

@@ -32,6 +32,7 @@ def _oracle_q(lts, order, steps, layer):
     (4, (0.5, 0.3, 0.2), "fp64", 1e-11),
     (5, (0.6, 0.25, 0.1, 0.05), "fp64", 1e-11),
     (3, (0.4, 0.35, 0.25), "fp32", 2e-5),
+    (8, (0.5, 0.3, 0.2), "fp64", 1e-11),
 ])
 def test_lts_matches_oracle(gpu, order, fractions, arith, tol):
     import torch

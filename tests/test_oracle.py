@@ -76,7 +76,7 @@ def _poly_step(order, dims=(6, 6, 6), relations_override=None, seed=3):
     return q0, q1, I, exact_q1, exact_I, inner
 
 
-@pytest.mark.parametrize("order", [2, 3, 4, 5, 6])
+@pytest.mark.parametrize("order", [2, 3, 4, 5, 6, 8])
 def test_one_step_exact_for_polynomial_states(order):
     q0, q1, I, exact_q1, exact_I, inner = _poly_step(order)
     assert len(inner) > 50
