@@ -96,15 +96,14 @@ struct Cfg {
   static constexpr int LDS_L = NX * kCols * LDX_L * 2 + kTileElems * 72 * 4 + 9 * 16 * kTileElems * 4 +
                                (scaled ? S::MT * kCols * 4 : 0);
   static constexpr int WAVES_L = (V != 3 && LDS_L <= 81920) ? 2 : 1;
-  static constexpr int LDX_N0 = S::KMAX + 8;
-  static constexpr int lds_n(int ldx) {
-    return NX * kCols * ldx * 2 + kTileElems * 9 * S::B2P * 4 + kTileElems * 81 * 4 + 3 * S::B2 * O * 8 +
-           (scaled ? kCols * 4 : 0);
-  }
-  // neighbour kernel: drop the padding when that makes two workgroups fit
-  static constexpr int LDX_N = (V != 3 && lds_n(LDX_N0) > 81920 && lds_n(S::KMAX) <= 81920) ? S::KMAX : LDX_N0;
-  static constexpr int WAVES_N = (V != 3 && lds_n(LDX_N) <= 81920) ? 2 : 1;
-  static constexpr int IPT = cdiv(kTileElems * S::B2P / 2, S::THREADS);  // face row pairs per thread
+  // neighbour kernel: one face's operand at a time (32 rows, 8 of padding in the stride)
+  static constexpr int LDX_N = 40;
+  static constexpr int LDS_N = NX * kCols * LDX_N * 2 + kTileElems * 9 * S::B2P * 4 + kTileElems * 81 * 4 +
+                               3 * S::B2 * O * 8 + (scaled ? kCols * 4 : 0) + 2 * kTileElems * 4 * 4;
+  static constexpr int WGS_N = 163840 / LDS_N;  // workgroups per CU that the LDS allows
+  // minimum waves per SIMD asked of the register allocator: three when the LDS
+  // lets three or more waves per SIMD run
+  static constexpr int WAVES_N = V == 3 ? 1 : (WGS_N * S::MT >= 12 ? 3 : 2);
 };
 
 __device__ inline floatx4 mma(bf16x8 a, bf16x8 b, floatx4 c) {
@@ -380,7 +379,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
   constexpr int B = S::B, B2 = S::B2, MT = S::MT, LDX = C::LDX_L, FR = S::FR, NX = C::NX;
   constexpr bool scaled = C::scaled;
   constexpr int TPW = cdiv(S::MTF, MT);
-  __shared__ T xs[NX * kCols * LDX];
+  __shared__ __attribute__((aligned(16))) T xs[NX * kCols * LDX];
   __shared__ float sStar[kTileElems * 72];
   __shared__ float sStage[3 * 3 * 16 * kTileElems];  // CK partial sums handed between waves (3 slots)
   __shared__ float sMax[scaled ? MT * kCols : 1];
@@ -684,19 +683,21 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   using S = MShape<O>;
   using C = Cfg<O, V>;
   using T = typename Prec<V>::T;
-  constexpr int B = S::B, B2 = S::B2, LDX = C::LDX_N, FR = S::FR, NX = C::NX, IPT = C::IPT;
+  constexpr int B = S::B, B2 = S::B2, B2P = S::B2P, LDX = C::LDX_N, NX = C::NX;
   constexpr bool scaled = C::scaled;
-  constexpr int HALF = S::B2P / 2;              // row pairs of one (padded) face
-  constexpr int ITEMS = kTileElems * HALF;
-  __shared__ T xs[NX * kCols * LDX];
-  __shared__ __attribute__((aligned(16))) float sTr[kTileElems * 9 * S::B2P];
+  constexpr int HALF = B2P / 2;                  // row pairs of one (padded) face
+  static_assert(HALF <= 16 && B2P <= 32 && LDX >= 32, "a face's row pairs fit a 16-lane group, its rows a k-step");
+  constexpr int GROUPS = S::THREADS / 16;        // 16-lane groups per pass, one element each
+  constexpr int PASSES = cdiv(kTileElems, GROUPS);
+  __shared__ __attribute__((aligned(16))) T xs[NX * kCols * LDX];  // face operand: rows [0, B2P) of one face, the rest zero
+  __shared__ __attribute__((aligned(16))) float sTr[kTileElems * 9 * B2P];
   __shared__ float sAm[kTileElems * 81];
   __shared__ float sFP[3 * B2 * O];   // fP_h rows: O (value, column) entries each
   __shared__ int sFPCol[3 * B2 * O];
-  __shared__ unsigned sMaxU[scaled ? kCols : 1];
+  __shared__ float sUn[scaled ? kCols : 1];  // unscale factor of each operand column of the current face
   __shared__ int sLink[kTileElems * 4];  // trace index nb * 4 + side of (element, face), -1: none
   __shared__ int sRel[kTileElems * 4];   // orientation h of (element, face)
-  constexpr int TR = 9 * S::B2P;                               // floats of one (padded) face trace
+  constexpr int TR = 9 * B2P;                                  // floats of one (padded) face trace
   constexpr int TR4 = TR / 4;                                  // 16-byte pieces of one face trace
   constexpr int GPT = cdiv(kTileElems * TR4, S::THREADS);      // gathered pieces per thread
   constexpr int APT = cdiv(kTileElems * 81, S::THREADS);       // AminusT floats per thread
@@ -705,8 +706,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));  // wave-uniform
   const int g = lane >> 4, e = lane & 15;
   const int64_t E0 = static_cast<int64_t>(blockIdx.x) * kTileElems;
-  const int64_t elem = E0 + e;
-  const bool valid = elem < n;
+  const bool valid = E0 + e < n;
   float* const tileDofs = dofs + E0 * 9 * B;
   const int laneQ = e * 9 * B + 16 * w + 4 * g;
 
@@ -726,14 +726,13 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
     sLink[threadIdx.x] = link;
     sRel[threadIdx.x] = h;
   }
-  if constexpr (scaled)
-    for (int i = threadIdx.x; i < kCols; i += blockDim.x) sMaxU[i] = 0u;
-  // zero padding rows of the face operand
-  // rows beyond the four faces (the face rows' padding is written as zeros by the items)
-  for (int i = threadIdx.x; i < kCols * (S::KPF - FR); i += blockDim.x) {
-    const int col = i / (S::KPF - FR), k = FR + i % (S::KPF - FR);
+  // rows [B2P, 32) of the operand stay zero (rows [B2, B2P) are written as zeros by the items)
+  if constexpr (B2P < 32) {
+    for (int i = threadIdx.x; i < kCols * (32 - B2P); i += blockDim.x) {
+      const int col = i / (32 - B2P), k = B2P + i % (32 - B2P);
 #pragma unroll
-    for (int pl = 0; pl < NX; ++pl) xs[pl * kCols * LDX + col * LDX + k] = static_cast<T>(0.0f);
+      for (int pl = 0; pl < NX; ++pl) xs[pl * kCols * LDX + col * LDX + k] = static_cast<T>(0.0f);
+    }
   }
   __syncthreads();
   // the next face's traces and AminusT are loaded into registers while the
@@ -759,11 +758,14 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
     }
   };
   gather(0);
-  // the fp16 form keeps the face operand in registers until its column scales are known
-  float wv[scaled ? 4 : 1][scaled ? IPT : 1][2][9];
+  floatx4 tot[1][9];
 #pragma unroll
+  for (int p = 0; p < 9; ++p) tot[0][p] = floatx4{0, 0, 0, 0};
+  // face by face: W_f = (fP_h Tr_f) AminusT_f into the operand (the fp16 form
+  // scales each column by a power of two), then Q += rDivM_f W_f as one k-step
+#pragma unroll 1
   for (int f = 0; f < 4; ++f) {
-    __syncthreads();
+    // the previous face's items have read sTr / sAm (barrier before its product)
 #pragma unroll
     for (int i = 0; i < GPT; ++i) {
       const int idx = threadIdx.x + i * S::THREADS;
@@ -775,90 +777,93 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
       if (idx < kTileElems * 81) sAm[idx] = av[i];
     }
     if (f < 3) gather(f + 1);
-    __syncthreads();
+    __syncthreads();  // also: every wave's product of the previous face has read xs and sUn
 #pragma unroll
-    for (int it = 0; it < IPT; ++it) {
-      const int item = threadIdx.x + it * S::THREADS;
-      if (item < ITEMS) {
-        // rows 2 rp and 2 rp + 1 of face f of element s (rows >= B2 are padding: zero)
-        const int s = item / HALF, r0 = 2 * (item % HALF);
-        const bool regular = sLink[s * 4 + f] >= 0;
-        const int h = sRel[s * 4 + f];
-        const float* tr = sTr + s * TR;
-        float u[2][9];
+    for (int ps = 0; ps < PASSES; ++ps) {
+      // rows 2 rp and 2 rp + 1 of face f of element s (rows >= B2 are padding: zero)
+      const int s = static_cast<int>(threadIdx.x >> 4) + ps * GROUPS;
+      const int rp = static_cast<int>(threadIdx.x & 15);
+      const int r0 = 2 * rp;
+      const bool act = s < kTileElems && rp < HALF;
+      const int sl = s < kTileElems ? s : 0;
+      const bool regular = act && sLink[sl * 4 + f] >= 0;
+      const int h = sRel[sl * 4 + f];
+      const float* tr = sTr + sl * TR;
+      float u[2][9];
 #pragma unroll
-        for (int ri = 0; ri < 2; ++ri) {
+      for (int ri = 0; ri < 2; ++ri) {
 #pragma unroll
-          for (int qq = 0; qq < 9; ++qq) u[ri][qq] = 0.0f;
-          const int row = r0 + ri;
-          if (row < B2) {
-            const float* fp = sFP + (h * B2 + row) * O;
-            const int* fc = sFPCol + (h * B2 + row) * O;
+        for (int qq = 0; qq < 9; ++qq) u[ri][qq] = 0.0f;
+        const int row = r0 + ri;
+        if (regular && row < B2) {
+          const float* fp = sFP + (h * B2 + row) * O;
+          const int* fc = sFPCol + (h * B2 + row) * O;
 #pragma unroll
-            for (int kk = 0; kk < O; ++kk) {
-              const float c = fp[kk];
-              const int mm = fc[kk];
+          for (int kk = 0; kk < O; ++kk) {
+            const float c = fp[kk];
+            const int mm = fc[kk];
 #pragma unroll
-              for (int qq = 0; qq < 9; ++qq) u[ri][qq] += c * tr[qq * S::B2P + mm];
-            }
+            for (int qq = 0; qq < 9; ++qq) u[ri][qq] += c * tr[qq * B2P + mm];
           }
         }
-        const float* A = sAm + s * 81;
+      }
+      const float* A = sAm + sl * 81;
+      float w0[9], w1[9];
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        float acc0 = 0.0f, acc1 = 0.0f;
+        if (regular)
+#pragma unroll
+          for (int qq = 0; qq < 9; ++qq) {
+            const float a = A[qq + 9 * p];
+            acc0 += u[0][qq] * a;
+            acc1 += u[1][qq] * a;
+          }
+        w0[p] = acc0;
+        w1[p] = acc1;
+      }
+      if constexpr (scaled) {
+        // column maxima over the 16 lanes of the element (inactive lanes hold zeros)
 #pragma unroll
         for (int p = 0; p < 9; ++p) {
-          float acc0 = 0.0f, acc1 = 0.0f;
-          if (regular)
-#pragma unroll
-            for (int qq = 0; qq < 9; ++qq) {
-              const float a = A[qq + 9 * p];
-              acc0 += u[0][qq] * a;
-              acc1 += u[1][qq] * a;
-            }
-          if constexpr (scaled) {
-            wv[f][it][0][p] = acc0;
-            wv[f][it][1][p] = acc1;
-            atomicMax(&sMaxU[p * 16 + s], __float_as_uint(fmaxf(fabsf(acc0), fabsf(acc1))));
-          } else {
-            storeX2<V, LDX>(xs, p * 16 + s, f * S::B2P + r0, acc0, acc1);
-          }
+          float m = fmaxf(fabsf(w0[p]), fabsf(w1[p]));
+          m = fmaxf(m, __shfl_xor(m, 1));
+          m = fmaxf(m, __shfl_xor(m, 2));
+          m = fmaxf(m, __shfl_xor(m, 4));
+          m = fmaxf(m, __shfl_xor(m, 8));
+          float sc, inv;
+          columnScale(m, sc, inv);
+          if (act) storeX2<V, LDX>(xs, p * 16 + s, r0, w0[p] * sc, w1[p] * sc);
+          if (s < kTileElems && rp == 0) sUn[p * 16 + s] = inv;
         }
+      } else {
+#pragma unroll
+        for (int p = 0; p < 9; ++p)
+          if (act) storeX2<V, LDX>(xs, p * 16 + s, r0, w0[p], w1[p]);
       }
-    }
-  }
-  __syncthreads();
-  float un[9];
-#pragma unroll
-  for (int p = 0; p < 9; ++p) un[p] = 1.0f;
-  if constexpr (scaled) {
-#pragma unroll
-    for (int f = 0; f < 4; ++f)
-#pragma unroll
-      for (int it = 0; it < IPT; ++it) {
-        const int item = threadIdx.x + it * S::THREADS;
-        if (item < ITEMS) {
-          const int s = item / HALF, r0 = 2 * (item % HALF);
-#pragma unroll
-          for (int p = 0; p < 9; ++p) {
-            float sc, inv;
-            columnScale(__uint_as_float(sMaxU[p * 16 + s]), sc, inv);
-            storeX2<V, LDX>(xs, p * 16 + s, f * S::B2P + r0, wv[f][it][0][p] * sc, wv[f][it][1][p] * sc);
-          }
-        }
-      }
-#pragma unroll
-    for (int p = 0; p < 9; ++p) {
-      float sc;
-      columnScale(__uint_as_float(sMaxU[p * 16 + e]), sc, un[p]);
     }
     __syncthreads();
-  }
-  floatx4 acc[1][9];
+    const uint16_t* a[1] = {frag + S::rdmF + (static_cast<int64_t>(w) * 4 + f) * kFragPair};
+    if constexpr (scaled) {
+      floatx4 acc[1][9];
 #pragma unroll
-  for (int p = 0; p < 9; ++p) acc[0][p] = floatx4{0, 0, 0, 0};
-  const uint16_t* a[1] = {frag + S::rdm + static_cast<int64_t>(w) * S::KSF * kFragPair};
-  gemmTile<V, 1, 0, 9, LDX>(acc, a, S::KSF, xs, lane);
+      for (int p = 0; p < 9; ++p) acc[0][p] = floatx4{0, 0, 0, 0};
+      gemmTile<V, 1, 0, 9, LDX>(acc, a, 1, xs, lane);
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        const float un = sUn[p * 16 + e];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tot[0][p][j] += acc[0][p][j] * un;
+      }
+    } else {
+      gemmTile<V, 1, 0, 9, LDX>(tot, a, 1, xs, lane);
+    }
+  }
+  float one[9];
+#pragma unroll
+  for (int p = 0; p < 9; ++p) one[p] = 1.0f;
   const float aUnR = tables[S::tAUn + 7];
-  if (valid) updateRows<B>(tileDofs + laneQ, 16 * w + 4 * g, acc[0], aUnR, un, nullptr);
+  if (valid) updateRows<B>(tileDofs + laneQ, 16 * w + 4 * g, tot[0], aUnR, one, nullptr);
 }
 
 template <int O, int V>

@@ -37,7 +37,9 @@ struct MShape {
   static constexpr int64_t vol = ck + 3LL * MT * KSB * kFragPair;    // [3][MT][KSB]
   static constexpr int64_t fmrt = vol + 3LL * MT * KSB * kFragPair;  // [MTF][KSB]
   static constexpr int64_t rdm = fmrt + 1LL * MTF * KSB * kFragPair; // [MT][KSF]
-  static constexpr int64_t total = rdm + 1LL * MT * KSF * kFragPair;
+  // rDivM_f one face at a time: k-step f holds face f's B2 columns, zero-padded to 32
+  static constexpr int64_t rdmF = rdm + 1LL * MT * KSF * kFragPair;  // [MT][4]
+  static constexpr int64_t total = rdmF + 4LL * MT * kFragPair;
   // float tables: invM2[B2], fP[3][B2][B2] as at most O non-zeros per row
   // (every fP_h row has at most O), unscale factors of the stationary
   // matrices [ck0, ck1, ck2, vol0, vol1, vol2, fmrt, rdm] (1 unless the

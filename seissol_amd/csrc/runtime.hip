@@ -154,6 +154,10 @@ int uploadMfmaData(EngineState& s, const double* gm) {
       S::MTF, S::KSB, S::fmrt, 6);
   add([&](int r, int c) { return c % B2P < B2 ? gm[g.rDivM + ((c / B2P) * B + r) * B2 + c % B2P] : 0.0; }, B, S::FR,
       S::MT, S::KSF, S::rdm, 7);
+  // the same matrix per face (same scale: the same non-zeros), for the neighbour kernel's face-by-face product
+  static_assert(B2P <= 32, "one face's rows fit one k-step");
+  add([&](int r, int c) { return c % 32 < B2 ? gm[g.rDivM + ((c / 32) * B + r) * B2 + c % 32] : 0.0; }, B, 128,
+      S::MT, 4, S::rdmF, 7);
   for (int m = 0; m < B2; ++m) tables[S::tInvM2 + m] = static_cast<float>(1.0 / gm[g.mass2 + m]);
   // fP_h rows as (value, column) lists of O entries, padded with zeros
   for (int h = 0; h < 3; ++h) {
