@@ -142,3 +142,23 @@ def test_proxy_cli_rejects_bad_arguments():
     assert r.returncode == 255 and "usage" in r.stderr
     r = subprocess.run([str(exe), "-f", "xml", "10", "1", "all"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 255
+
+
+def test_friction_layer_checks_need_no_device():
+    # sizes are checked and an empty fault layer returns before any device call
+    lib = _capi.lib()
+    prm = _capi.FrictionParams()
+    dt = np.full(4, 0.25e-3)
+    tw = np.full(4, 0.25e-3)
+    layer = _capi.FaultLayer(0, 25, 25, 4, 0)
+    assert lib.sdg_friction_evaluate(ctypes.byref(prm), ctypes.byref(layer), 0.0, dt.ctypes.data, tw.ctypes.data,
+                                     None, None, None) == 0
+    for bad in (_capi.FaultLayer(0, 65, 65, 4, 0), _capi.FaultLayer(0, 16, 25, 4, 0), _capi.FaultLayer(0, 25, 25, 9, 0),
+                _capi.FaultLayer(-1, 25, 25, 4, 0)):
+        assert lib.sdg_friction_evaluate(ctypes.byref(prm), ctypes.byref(bad), 0.0, dt.ctypes.data, tw.ctypes.data,
+                                         None, None, None) != 0
+        assert b"invalid layer sizes" in lib.sdg_last_error()
+    layer1 = _capi.FaultLayer(1, 25, 25, 4, 0)  # arrays missing
+    assert lib.sdg_friction_evaluate(ctypes.byref(prm), ctypes.byref(layer1), 0.0, dt.ctypes.data, tw.ctypes.data,
+                                     None, None, None) != 0
+    assert b"missing" in lib.sdg_last_error()

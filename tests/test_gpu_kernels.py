@@ -201,6 +201,14 @@ def test_empty_layer_and_errors(gpu):
     bad = _capi.Layer(10, None, None, None, None, None, None, None, None, None)
     assert _capi.lib().sdg_compute_batched_neighbors_integral(eng.handle, ctypes.byref(bad), None) != 0
     assert b"missing" in _capi.lib().sdg_last_error()
+    # an empty layer is a no-op for every form, the matrix-core kernels included
+    for arith in ("fp32", "bf16x3", "f16x3"):
+        e2 = Engine(4, arith)
+        _capi.check(_capi.lib().sdg_compute_batched_ader_local(e2.handle, coeffs.ctypes.data, 1.0,
+                                                                ctypes.byref(empty), None))
+        _capi.check(_capi.lib().sdg_compute_batched_neighbors_integral(e2.handle, ctypes.byref(empty), None))
+        _capi.check(_capi.lib().sdg_compute_batched_ader(e2.handle, coeffs.ctypes.data, 1.0, ctypes.byref(empty),
+                                                         None))
 
 
 # ---------------------------------------------------------------------------
