@@ -200,23 +200,50 @@ __device__ inline void columnScale(float cm, float& scale, float& unscale) {
   }
 }
 
+// Lane reductions in registers (no LDS round trip); every lane of the wave
+// must be active.
+// max over the wave's four 16-lane rows (lanes e, e + 16, e + 32, e + 48):
+// gfx950's row swaps exchange rows 0-1 with 2-3, then rows 0 with 1 and 2 with 3
+__device__ inline float rowsMax4(float m) {
+  const unsigned u = __float_as_uint(m);
+  const auto a = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+  m = fmaxf(__uint_as_float(a[0]), __uint_as_float(a[1]));
+  const unsigned u2 = __float_as_uint(m);
+  const auto b = __builtin_amdgcn_permlane16_swap(u2, u2, false, false);
+  return fmaxf(__uint_as_float(b[0]), __uint_as_float(b[1]));
+}
+// max over each 16-lane row through DPP: partners lane ^ 1 and lane ^ 2
+// (quad_perm), then the mirrored lane within 8 and within 16 lanes
+template <int CTRL>
+__device__ inline float dppMove(float m) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(m), __float_as_int(m), CTRL, 0xF, 0xF, false));
+}
+__device__ inline float rowMax16(float m) {
+  m = fmaxf(m, dppMove<0xB1>(m));   // quad_perm [1, 0, 3, 2]
+  m = fmaxf(m, dppMove<0x4E>(m));   // quad_perm [2, 3, 0, 1]
+  m = fmaxf(m, dppMove<0x141>(m));  // row_half_mirror
+  return fmaxf(m, dppMove<0x140>(m));  // row_mirror
+}
+
 // largest magnitude of each quantity column over the rows this wave holds
 // (lanes of one element differ in lane / 16), written to sMax[w][16 q + e]
 template <int R>
 __device__ inline void waveColumnMax(const float (&v)[R][9][4], bool has, float* sMax, int w, int lane) {
   const int g = lane >> 4, e = lane & 15;
+  float m[9];
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
-    float m = 0.0f;
+    m[q] = 0.0f;
     if (has)
 #pragma unroll
       for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) m = fmaxf(m, fabsf(v[r][q][j]));
-    m = fmaxf(m, __shfl_xor(m, 16));
-    m = fmaxf(m, __shfl_xor(m, 32));
-    if (g == 0) sMax[w * kCols + q * 16 + e] = m;
+        for (int j = 0; j < 4; ++j) m[q] = fmaxf(m[q], fabsf(v[r][q][j]));
+    m[q] = rowsMax4(m[q]);
   }
+  if (g == 0)
+#pragma unroll
+    for (int q = 0; q < 9; ++q) sMax[w * kCols + q * 16 + e] = m[q];
 }
 
 // column scales of this lane's element from the maxima of the WAVES waves
@@ -823,14 +850,10 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
         w1[p] = acc1;
       }
       if constexpr (scaled) {
-        // column maxima over the 16 lanes of the element (inactive lanes hold zeros)
+        // column maxima over the 16 lanes of the element (idle lanes hold zeros)
 #pragma unroll
         for (int p = 0; p < 9; ++p) {
-          float m = fmaxf(fabsf(w0[p]), fabsf(w1[p]));
-          m = fmaxf(m, __shfl_xor(m, 1));
-          m = fmaxf(m, __shfl_xor(m, 2));
-          m = fmaxf(m, __shfl_xor(m, 4));
-          m = fmaxf(m, __shfl_xor(m, 8));
+          const float m = rowMax16(fmaxf(fabsf(w0[p]), fabsf(w1[p])));
           float sc, inv;
           columnScale(m, sc, inv);
           if (act) storeX2<V, LDX>(xs, p * 16 + s, r0, w0[p] * sc, w1[p] * sc);
