@@ -128,28 +128,63 @@ __device__ inline void applyStars(const real (&acc)[3][R][9], const real* sS, re
   }
 }
 
-// acc[j][r][p] += sum_{l<K} AT_j[l][row_r] * X[l][p], j < J, rows < M
+// acc[j][r][p] += sum_{l<K} AT_j[l][row_r] * X[l][p], j < J, rows < M.
+// Two steps per iteration with two register sets: the entries of steps l + 2
+// and l + 3 are loaded while steps l and l + 1 are computed, so that a wave
+// does not wait on the cache once per step.
 template <int J, int R, typename real>
 __device__ inline void gemmRows(real (&acc)[J][R][9], const real* __restrict__ AT, int64_t strideJ, int ldAT,
                                 const real* X, int K, int t, int tpe, int M) {
-  for (int l = 0; l < K; ++l) {
+  if (K <= 0) return;
+  int rr[R];
+  bool ok[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int row = t + r * tpe;
+    // clamped address: lanes past the last row read a valid entry and drop it
+    ok[r] = row < M;
+    rr[r] = ok[r] ? row : M - 1;
+  }
+  auto load = [&](int l, real (&v)[J][R]) {
+    const int lc = l < K ? l : K - 1;  // steps past the end reload the last one (in bounds, unused)
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[j][r] = AT[j * strideJ + static_cast<int64_t>(lc) * ldAT + rr[r]];
+  };
+  auto step = [&](int l, const real (&v)[J][R]) {
     real x[9];
 #pragma unroll
     for (int p = 0; p < 9; ++p) x[p] = X[l * 9 + p];
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
+    for (int j = 0; j < J; ++j)
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int row = t + r * tpe;
-        // clamped address: lanes past the last row read a valid entry and drop it
-        const int rr = row < M ? row : M - 1;
-        const real av = AT[j * strideJ + static_cast<int64_t>(l) * ldAT + rr];
-        const real a = row < M ? av : real(0);
+        const real a = ok[r] ? v[j][r] : real(0);
 #pragma unroll
         for (int p = 0; p < 9; ++p) acc[j][r][p] += a * x[p];
       }
-    }
+  };
+  real a0[J][R], a1[J][R];
+  load(0, a0);
+  load(1, a1);
+  int l = 0;
+  for (; l + 1 < K; l += 2) {
+    real b0[J][R], b1[J][R];
+    load(l + 2, b0);
+    load(l + 3, b1);
+    __builtin_amdgcn_sched_barrier(0);
+    step(l, a0);
+    step(l + 1, a1);
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        a0[j][r] = b0[j][r];
+        a1[j][r] = b1[j][r];
+      }
   }
+  if (l < K) step(l, a0);
 }
 
 // ---------------------------------------------------------------------------
