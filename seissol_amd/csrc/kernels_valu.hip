@@ -208,17 +208,17 @@ __device__ inline void gemmRows(real (&acc)[J][R][9], const real* __restrict__ A
   if (l < K) step(l, a0);
 }
 
-// the two-step products where the kernel keeps them without register spills
-// (checked on the compiled kernels): FP64 at ORDER 6 (the neighbour kernel
-// with its default tile only) and at ORDER 5 with the tile of more rows per
-// lane; FP32 except ORDER 5
+// the two-step products where they were measured faster and the kernel stays
+// free of register spills: ORDER 6 (FP64: the neighbour kernel with its
+// default tile only) and FP64 ORDER 5 with the tile of more rows per lane
+// (ORDER 8 FP32 was slower with them)
 template <typename real, int O, int RX, bool Neighbor>
 constexpr bool deepProducts() {
   if constexpr (sizeof(real) == 8) {
     if (O == 6) return RX == 0 || !Neighbor;
     return O == 5 && RX == 1;
   } else {
-    return O != 5;
+    return O == 6;
   }
 }
 
