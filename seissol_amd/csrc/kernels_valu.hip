@@ -129,14 +129,36 @@ __device__ inline void applyStars(const real (&acc)[3][R][9], const real* sS, re
 }
 
 // acc[j][r][p] += sum_{l<K} AT_j[l][row_r] * X[l][p], j < J, rows < M.
-// Deep: two steps per iteration with two register sets, the entries of steps
-// l + 2 and l + 3 loaded while steps l and l + 1 are computed, so that a wave
-// does not wait on the cache once per step; else one step at a time.
-template <int J, int R, bool Deep, typename real>
+// Mode 2: two steps per iteration with two register sets, the entries of
+// steps l + 2 and l + 3 loaded while steps l and l + 1 are computed, so that
+// a wave does not wait on the cache once per step; else one step at a time,
+// unrolled as the compiler chooses (mode 0) or not unrolled (mode 1).
+template <int J, int R, int Mode, typename real>
 __device__ inline void gemmRows(real (&acc)[J][R][9], const real* __restrict__ AT, int64_t strideJ, int ldAT,
                                 const real* X, int K, int t, int tpe, int M) {
   if (K <= 0) return;
-  if constexpr (!Deep) {
+  if constexpr (Mode == 1) {
+#pragma unroll 1
+    for (int l = 0; l < K; ++l) {
+      real x[9];
+#pragma unroll
+      for (int p = 0; p < 9; ++p) x[p] = X[l * 9 + p];
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int row = t + r * tpe;
+          const int rr = row < M ? row : M - 1;
+          const real av = AT[j * strideJ + static_cast<int64_t>(l) * ldAT + rr];
+          const real a = row < M ? av : real(0);
+#pragma unroll
+          for (int p = 0; p < 9; ++p) acc[j][r][p] += a * x[p];
+        }
+      }
+    }
+    return;
+  }
+  if constexpr (Mode == 0) {
     for (int l = 0; l < K; ++l) {
       real x[9];
 #pragma unroll
@@ -208,17 +230,20 @@ __device__ inline void gemmRows(real (&acc)[J][R][9], const real* __restrict__ A
   if (l < K) step(l, a0);
 }
 
-// the two-step products where they were measured faster and the kernel stays
-// free of register spills: ORDER 6 (FP64: the neighbour kernel with its
-// default tile only) and FP64 ORDER 5 with the tile of more rows per lane
-// (ORDER 8 FP32 was slower with them)
+// product loop of gemmRows per kernel, each choice measured faster than the
+// others where it is used: two-step loads (2) at ORDER 6 (FP64: the neighbour
+// kernel with its default tile only, the other tile spills) and for FP64
+// ORDER 5 with the tile of more rows per lane; FP32 at ORDER <= 5 not unrolled
+// (1); else one step at a time as the compiler unrolls it (0).  The FP64
+// ORDER 4 kernels then spill about 0.9 KB per lane and are still faster than
+// their spill-free rolled form.
 template <typename real, int O, int RX, bool Neighbor>
-constexpr bool deepProducts() {
+constexpr int productMode() {
   if constexpr (sizeof(real) == 8) {
-    if (O == 6) return RX == 0 || !Neighbor;
-    return O == 5 && RX == 1;
+    if (O == 6) return (RX == 0 || !Neighbor) ? 2 : 0;
+    return (O == 5 && RX == 1) ? 2 : 0;
   } else {
-    return O == 6;
+    return O == 6 ? 2 : (O <= 5 ? 1 : 0);
   }
 }
 
@@ -233,7 +258,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
                                              const uint8_t* __restrict__ faceTypes, LtsArgs lts) {
   using S = Shape<O, RX, WV>;
   constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
-  constexpr bool kDeep = deepProducts<real, O, RX, false>();
+  constexpr int kProd = productMode<real, O, RX, false>();
   __shared__ real sA[S::EPB * S::QS];
   __shared__ real sB[S::EPB * (S::FS > S::QS ? S::FS : S::QS)];
   __shared__ real sStar[S::EPB * 3 * kStarNnz];
@@ -302,7 +327,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int p = 0; p < 9; ++p) acc[j][r][p] = real(0);
-    gemmRows<3, R, kDeep>(acc, mats + S::ck, static_cast<int64_t>(B) * B, B, src, nIn, t, TPE, m);
+    gemmRows<3, R, kProd>(acc, mats + S::ck, static_cast<int64_t>(B) * B, B, src, nIn, t, TPE, m);
     real dq[R][9];
     applyStars<R>(acc, sS, dq);
     __syncthreads();  // all reads of dst (previous src) done
@@ -355,7 +380,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
         for (int r = 0; r < R; ++r)
 #pragma unroll
           for (int p = 0; p < 9; ++p) acc[j][r][p] = real(0);
-      gemmRows<3, R, kDeep>(acc, mats + S::vol, static_cast<int64_t>(B) * B, B, xa, B, t, TPE, B);
+      gemmRows<3, R, kProd>(acc, mats + S::vol, static_cast<int64_t>(B) * B, B, xa, B, t, TPE, B);
       real y[R][9];
       applyStars<R>(acc, sS, y);
 #pragma unroll
@@ -371,7 +396,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int p = 0; p < 9; ++p) tr[0][r][p] = real(0);
-      gemmRows<1, R, kDeep>(tr, mats + S::fmrt + static_cast<int64_t>(f) * B * B2, 0, B2, xa, B, t, TPE, B2);
+      gemmRows<1, R, kProd>(tr, mats + S::fmrt + static_cast<int64_t>(f) * B * B2, 0, B2, xa, B, t, TPE, B2);
       const bool skip = !active || (faceTypes != nullptr && faceTypes[e * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE);
       const real* A = aplus + (active ? e : 0) * 324 + 81 * f;
 #pragma unroll
@@ -396,7 +421,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int p = 0; p < 9; ++p) acc[0][r][p] = qreg[r][p];
-      gemmRows<1, R, kDeep>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
+      gemmRows<1, R, kProd>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
       if (active) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -420,7 +445,7 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
                                               const uint8_t* __restrict__ faceTypes) {
   using S = Shape<O>;
   constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
-  constexpr bool kDeep = deepProducts<real, O, 0, false>();
+  constexpr int kProd = productMode<real, O, 0, false>();
   __shared__ real sA[S::EPB * S::QS];
   __shared__ real sB[S::EPB * S::FS];
   __shared__ real sStar[S::EPB * 3 * kStarNnz];
@@ -460,7 +485,7 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int p = 0; p < 9; ++p) acc[j][r][p] = real(0);
-    gemmRows<3, R, kDeep>(acc, mats + S::vol, static_cast<int64_t>(B) * B, B, xa, B, t, TPE, B);
+    gemmRows<3, R, kProd>(acc, mats + S::vol, static_cast<int64_t>(B) * B, B, xa, B, t, TPE, B);
     real y[R][9];
     applyStars<R>(acc, sS, y);
 #pragma unroll
@@ -475,7 +500,7 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
     for (int r = 0; r < R; ++r)
 #pragma unroll
       for (int p = 0; p < 9; ++p) tr[0][r][p] = real(0);
-    gemmRows<1, R, kDeep>(tr, mats + S::fmrt + static_cast<int64_t>(f) * B * B2, 0, B2, xa, B, t, TPE, B2);
+    gemmRows<1, R, kProd>(tr, mats + S::fmrt + static_cast<int64_t>(f) * B * B2, 0, B2, xa, B, t, TPE, B2);
     const bool skip = !active || (faceTypes != nullptr && faceTypes[e * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE);
     const real* A = aplus + (active ? e : 0) * 324 + 81 * f;
 #pragma unroll
@@ -499,7 +524,7 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
   for (int r = 0; r < R; ++r)
 #pragma unroll
     for (int p = 0; p < 9; ++p) acc[0][r][p] = qreg[r][p];
-  gemmRows<1, R, kDeep>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
+  gemmRows<1, R, kProd>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
   if (active) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -523,7 +548,7 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
                                                  const uint8_t* __restrict__ faceTypes, NeighborLts<O, real> lts) {
   using S = Shape<O, RX, WV>;
   constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
-  constexpr bool kDeep = deepProducts<real, O, RX, true>();
+  constexpr int kProd = productMode<real, O, RX, true>();
   __shared__ real sA[S::EPB * S::QS];
   __shared__ real sU[S::EPB * B2 * 9];
   __shared__ real sB[S::EPB * S::FS];
@@ -578,7 +603,7 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int p = 0; p < 9; ++p) u[0][r][p] = real(0);
-      gemmRows<1, R, kDeep>(u, mats + S::rt + static_cast<int64_t>(j) * B * B2, 0, B2, xa, B, t, TPE, B2);
+      gemmRows<1, R, kProd>(u, mats + S::rt + static_cast<int64_t>(j) * B * B2, 0, B2, xa, B, t, TPE, B2);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int row = t + r * TPE;
@@ -594,7 +619,7 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int p = 0; p < 9; ++p) v[0][r][p] = real(0);
-      gemmRows<1, R, kDeep>(v, mats + S::fp + static_cast<int64_t>(h) * B2 * B2, 0, B2, xu, B2, t, TPE, B2);
+      gemmRows<1, R, kProd>(v, mats + S::fp + static_cast<int64_t>(h) * B2 * B2, 0, B2, xu, B2, t, TPE, B2);
       const real* A = aminus + (active ? e : 0) * 324 + 81 * f;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -618,7 +643,7 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
   for (int r = 0; r < R; ++r)
 #pragma unroll
     for (int p = 0; p < 9; ++p) acc[0][r][p] = real(0);
-  gemmRows<1, R, kDeep>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
+  gemmRows<1, R, kProd>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
   if (active) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
