@@ -130,7 +130,23 @@ __global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, c
     if (elem < 0) continue;
     const int64_t e = elem;
     waveSync();  // the previous side's reads of sD/sT are done
-    for (int i = lane; i < DS; i += 64) sD[wave][i] = deriv[e * dsize + i];
+    // the element's derivatives in chunks of six loads per lane, each chunk
+    // issued before its first value is stored (clamped addresses, in bounds)
+    constexpr int DI = (DS + 63) / 64;
+#pragma unroll
+    for (int c = 0; c < DI; c += 6) {
+      real tmp[6];
+#pragma unroll
+      for (int u = 0; u < 6; ++u) {
+        const int i = lane + 64 * (c + u);
+        tmp[u] = deriv[e * dsize + (i < DS ? i : DS - 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < 6; ++u) {
+        const int i = lane + 64 * (c + u);
+        if (c + u < DI && i < DS) sD[wave][i] = tmp[u];
+      }
+    }
     for (int i = lane; i < 81; i += 64) sT[wave][i] = static_cast<real>(L.tinv_t[f * 81 + i]);
     waveSync();
     const int k = lane;
