@@ -578,13 +578,30 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
     const unsigned setup = (lts.setup != nullptr && regular) ? lts.setup[e] : 0u;
     const bool fromDerivatives = (setup >> f) & 1u;
     const Coeffs<O, real>& cf = ((setup >> (4 + f)) & 1u) ? lts.gts : lts.sub;
-    if (laneOk) {
+    if (laneOk && !fromDerivatives) {
+      // the neighbour's buffer (zeros off regular faces) in chunks of six
+      // loads per lane, each chunk issued before its first value is stored
+      // (clamped addresses, in bounds; nb = 0 off regular faces)
+      constexpr int GI = (S::QS + TPE - 1) / TPE;
+#pragma unroll
+      for (int c = 0; c < GI; c += 6) {
+        real gv[6];
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+          const int idx = t + (c + u) * TPE;
+          gv[u] = integ[nb * S::QS + (idx < S::QS ? idx : S::QS - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+          const int idx = t + (c + u) * TPE;
+          if (c + u < GI && idx < S::QS) xa[(idx % B) * 9 + idx / B] = regular ? gv[u] : real(0);
+        }
+      }
+    } else if (laneOk) {
       for (int idx = t; idx < S::QS; idx += TPE) {
         const int p = idx / B, l = idx % B;
         real v = real(0);
-        if (regular && !fromDerivatives) {
-          v = integ[nb * S::QS + idx];
-        } else if (regular) {
+        if (regular) {
           int64_t off = nb * lts.dsize;
 #pragma unroll
           for (int i = 0; i < O; ++i) {
