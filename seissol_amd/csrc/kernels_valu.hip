@@ -247,6 +247,34 @@ constexpr int productMode() {
   }
 }
 
+// a tile's [EPB][9][B] block of src (elements e0, e0 + 1, ...) into LDS as
+// [element][row][quantity], zeros past element n - 1; the loads go in chunks
+// of six per thread, each chunk issued before its first value is stored
+// (clamped addresses, in bounds: a launched tile has e0 < n)
+template <int EPB, int QS, int B, int THREADS, typename real>
+__device__ inline void stageBlock(real* sA, const real* __restrict__ src, int64_t e0, int64_t n) {
+  constexpr int TOT = EPB * QS;
+  constexpr int NI = (TOT + THREADS - 1) / THREADS;
+  const int64_t last = n * QS - 1;
+#pragma unroll
+  for (int c = 0; c < NI; c += 6) {
+    real v[6];
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+      const int64_t g = e0 * QS + threadIdx.x + (c + u) * THREADS;
+      v[u] = src[g < last ? g : last];
+    }
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+      const int idx = static_cast<int>(threadIdx.x) + (c + u) * THREADS;
+      if (c + u < NI && idx < TOT) {
+        const int s = idx / QS, w = idx % QS;
+        sA[s * QS + (w % B) * 9 + w / B] = e0 + s < n ? v[u] : real(0);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // ader (+ optional volume and local flux): Time.cpp:65-140 and Local.cpp:124-160
 // ---------------------------------------------------------------------------
@@ -280,12 +308,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
   const bool accumulate = lts.setup != nullptr && !lts.reset && (setup & SDG_LTS_ACCUMULATE_BUFFERS);
 
   // stage Q as [row][quantity]
-  for (int idx = threadIdx.x; idx < S::EPB * S::QS; idx += blockDim.x) {
-    const int s = idx / S::QS, w = idx % S::QS;
-    const int p = w / B, l = w % B;
-    const int64_t ee = e0 + s;
-    sA[s * S::QS + l * 9 + p] = (ee < n) ? dofs[ee * S::QS + w] : real(0);
-  }
+  stageBlock<S::EPB, S::QS, B, S::THREADS>(sA, dofs, e0, n);
   loadStars(sStar, star, e0, n, S::EPB);
   __syncthreads();
 
@@ -457,12 +480,7 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
   const bool laneOk = lane < S::EPW * TPE;
   const int64_t e = e0 + slot;
   const bool active = laneOk && e < n;
-  for (int idx = threadIdx.x; idx < S::EPB * S::QS; idx += blockDim.x) {
-    const int s = idx / S::QS, w = idx % S::QS;
-    const int p = w / B, l = w % B;
-    const int64_t ee = e0 + s;
-    sA[s * S::QS + l * 9 + p] = (ee < n) ? integ[ee * S::QS + w] : real(0);
-  }
+  stageBlock<S::EPB, S::QS, B, S::THREADS>(sA, integ, e0, n);
   loadStars(sStar, star, e0, n, S::EPB);
   __syncthreads();
   const int sl = laneOk ? slot : 0;
