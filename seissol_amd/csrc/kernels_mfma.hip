@@ -80,6 +80,10 @@ __device__ constexpr int kStarQ[9][3] = {{6, 7, 8}, {6, 7, 8}, {6, 7, 8}, {6, 7,
                                          {6, 8, -1}, {0, 3, 5}, {1, 3, 4}, {2, 4, 5}};
 __device__ constexpr int kStarOff[9] = {0, 3, 6, 9, 11, 13, 15, 18, 21};
 __device__ constexpr int kStarCnt[9] = {3, 3, 3, 2, 2, 2, 3, 3, 3};
+// entry z of a compact star matrix: its column-major position q + 9 p
+__device__ constexpr int kStarPos[24] = {6 + 0,  7 + 0,  8 + 0,  6 + 9,  7 + 9,  8 + 9,  6 + 18, 7 + 18,
+                                         8 + 18, 6 + 27, 7 + 27, 7 + 36, 8 + 36, 6 + 45, 8 + 45, 0 + 54,
+                                         3 + 54, 5 + 54, 1 + 63, 3 + 63, 4 + 63, 2 + 72, 4 + 72, 5 + 72};
 
 template <int O>
 struct CoeffsF {
@@ -427,12 +431,23 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
   const float aUnVol[3] = {aUn[3], aUn[4], aUn[5]};
   const float aUnF = aUn[6], aUnR = aUn[7];
 
-  for (int i = threadIdx.x; i < kTileElems * 72; i += blockDim.x) {
-    const int s = i / 72, r = i % 72, jj = r / 24;
-    int z = r % 24, p = 0;
-    while (z >= kStarCnt[p]) { z -= kStarCnt[p]; ++p; }
-    const int q = kStarQ[p][z];
-    sStar[i] = (E0 + s < n) ? star[E0 * 243 + s * 243 + 81 * jj + q + 9 * p] : 0.0f;
+  {
+    // the tile's compact star matrices, every load of a thread issued before
+    // the first store (clamped addresses, in bounds: a launched tile has E0 < n)
+    constexpr int SI = cdiv(kTileElems * 72, S::THREADS);
+    float sv[SI];
+#pragma unroll
+    for (int u = 0; u < SI; ++u) {
+      const int i = threadIdx.x + u * S::THREADS;
+      const int s = min(i / 72, kTileElems - 1), r = i % 72;
+      const int64_t el = E0 + s < n ? E0 + s : n - 1;
+      sv[u] = star[el * 243 + 81 * (r / 24) + kStarPos[r % 24]];
+    }
+#pragma unroll
+    for (int u = 0; u < SI; ++u) {
+      const int i = threadIdx.x + u * S::THREADS;
+      if (i < kTileElems * 72) sStar[i] = E0 + i / 72 < n ? sv[u] : 0.0f;
+    }
   }
 
   // Q itself is read again at the end, so only the time integral stays live
