@@ -616,18 +616,22 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
         }
       }
     } else if (laneOk) {
+      // the neighbour's derivatives integrated over the interval: the O
+      // loads of an entry are issued together, branch-free (an entry past a
+      // block's rows reads the block's first entry, in bounds, with weight 0)
       for (int idx = t; idx < S::QS; idx += TPE) {
         const int p = idx / B, l = idx % B;
-        real v = real(0);
-        if (regular) {
-          int64_t off = nb * lts.dsize;
+        real dv[O];
+        int64_t off = nb * lts.dsize;
 #pragma unroll
-          for (int i = 0; i < O; ++i) {
-            const int m = basis3(O - i);
-            if (l < m) v += cf.c[i] * lts.deriv[off + p * m + l];
-            off += 9 * m;
-          }
+        for (int i = 0; i < O; ++i) {
+          const int m = basis3(O - i);
+          dv[i] = lts.deriv[off + (l < m ? p * m + l : 0)];
+          off += 9 * m;
         }
+        real v = real(0);
+#pragma unroll
+        for (int i = 0; i < O; ++i) v += (l < basis3(O - i) ? static_cast<real>(cf.c[i]) : real(0)) * dv[i];
         xa[l * 9 + p] = v;
       }
     }
