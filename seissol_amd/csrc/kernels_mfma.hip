@@ -573,7 +573,8 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     for (int p = 0; p < 9; ++p) storeZero4<V, LDX>(xs, p * 16 + e, 16 * t + 4 * g);
   __syncthreads();
 
-  // face traces T = fMrT_f I (rows f * B2 + m)
+  // face traces rT_f I = (fMrT_f / M2) I (rows f * B2P + m): the fragments
+  // carry 1/M2, and the local flux's rDivM fragments carry M2
   float tr[TPW][9][4];
 #pragma unroll
   for (int ti = 0; ti < TPW; ++ti) {
@@ -592,19 +593,16 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
       for (int p = 0; p < 9; ++p)
 #pragma unroll
         for (int j = 0; j < 4; ++j) tr[ti][p][j] = acc[0][p][j] * (aUnF * un[p]);
-      // neighbour-side traces rT_f I = T / M2: the lane's four rows lie in one
+      // the traces: the lane's four rows lie in one
       // face (faces padded to B2P rows, a multiple of 4), one 16-byte store
       // per quantity into the padded layout [element][face][quantity][B2P]
       const int r0 = 16 * t + 4 * g;
       if (valid && r0 < FR) {
         const int f = r0 / S::B2P, mm = r0 % S::B2P;
-        float sc[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) sc[j] = mm + j < B2 ? tables[S::tInvM2 + mm + j] : 0.0f;
 #pragma unroll
         for (int p = 0; p < 9; ++p)
           *reinterpret_cast<float4*>(tileTraces + (e * 36 + f * 9 + p) * S::B2P + mm) =
-              float4{tr[ti][p][0] * sc[0], tr[ti][p][1] * sc[1], tr[ti][p][2] * sc[2], tr[ti][p][3] * sc[3]};
+              float4{tr[ti][p][0], tr[ti][p][1], tr[ti][p][2], tr[ti][p][3]};
       }
     }
   }
@@ -627,7 +625,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
 
   // Once every wave has finished the volume GEMM, the X area is free until W
   // is stored into it: it takes the four faces' AplusT and (when it fits) the
-  // volume term, so that W = T AplusT_f is formed by every lane at once with
+  // volume term, so that W = (rT_f I) AplusT_f is formed by every lane at once with
   // its own face's matrix (a lane's four rows of a tile lie in one face).
   constexpr int kXBytes = NX * kCols * LDX * 2;
   constexpr bool kParkVol = (kTileElems * 324 + kTileElems * 9 * B) * 4 <= kXBytes;
@@ -900,7 +898,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   float one[9];
 #pragma unroll
   for (int p = 0; p < 9; ++p) one[p] = 1.0f;
-  const float aUnR = tables[S::tAUn + 7];
+  const float aUnR = tables[S::tAUn + 8];
   if (valid) updateRows<B>(tileDofs + laneQ, 16 * w + 4 * g, tot[0], aUnR, one, nullptr);
 }
 

@@ -42,13 +42,13 @@ struct MShape {
   static constexpr int64_t total = rdmF + 4LL * MT * kFragPair;
   // float tables: invM2[B2], fP[3][B2][B2] as at most O non-zeros per row
   // (every fP_h row has at most O), unscale factors of the stationary
-  // matrices [ck0, ck1, ck2, vol0, vol1, vol2, fmrt, rdm] (1 unless the
-  // fragments were scaled by a power of two)
+  // matrices [ck0, ck1, ck2, vol0, vol1, vol2, fmrt, rdm, rdmF] (1 unless
+  // the fragments were scaled by a power of two)
   static constexpr int tInvM2 = 0;
   static constexpr int tFP = B2;                  // fP_h rows as O (value) entries: [3][B2][O]
   static constexpr int tFPCol = tFP + 3 * B2 * O; // their columns (exact small integers): [3][B2][O]
   static constexpr int tAUn = tFPCol + 3 * B2 * O;
-  static constexpr int tTotal = tAUn + 8;
+  static constexpr int tTotal = tAUn + 9;
 };
 
 }  // namespace mfma

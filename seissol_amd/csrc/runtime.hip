@@ -148,16 +148,25 @@ int uploadMfmaData(EngineState& s, const double* gm) {
     add([&](int r, int c) { return gm[g.kDivM + (j * B + r) * B + c]; }, B, B, S::MT, S::KSB,
         S::vol + static_cast<int64_t>(j) * S::MT * S::KSB * mfma::kFragPair, 3 + j);
   }
-  // the four face operands stacked with B2P rows each (rows B2..B2P-1 zero)
+  // the four face operands stacked with B2P rows each (rows B2..B2P-1 zero).
+  // The local pass forms the traces rT_f I = (fMrT_f / M2) I directly, so its
+  // flux matrices carry the mass factor instead: rDivM_f diag(M2) (the
+  // neighbour pass uses rDivM_f itself, per face, below).
   constexpr int B2P = S::B2P;
-  add([&](int r, int c) { return r % B2P < B2 ? gm[g.fMrT + ((r / B2P) * B2 + r % B2P) * B + c] : 0.0; }, S::FR, B,
-      S::MTF, S::KSB, S::fmrt, 6);
-  add([&](int r, int c) { return c % B2P < B2 ? gm[g.rDivM + ((c / B2P) * B + r) * B2 + c % B2P] : 0.0; }, B, S::FR,
-      S::MT, S::KSF, S::rdm, 7);
-  // the same matrix per face (same scale: the same non-zeros), for the neighbour kernel's face-by-face product
+  add([&](int r, int c) {
+        const int m = r % B2P;
+        return m < B2 ? gm[g.fMrT + ((r / B2P) * B2 + m) * B + c] / gm[g.mass2 + m] : 0.0;
+      },
+      S::FR, B, S::MTF, S::KSB, S::fmrt, 6);
+  add([&](int r, int c) {
+        const int m = c % B2P;
+        return m < B2 ? gm[g.rDivM + ((c / B2P) * B + r) * B2 + m] * gm[g.mass2 + m] : 0.0;
+      },
+      B, S::FR, S::MT, S::KSF, S::rdm, 7);
+  // rDivM_f per face (k-step f: face f's B2 columns), for the neighbour kernel's face-by-face product
   static_assert(B2P <= 32, "one face's rows fit one k-step");
   add([&](int r, int c) { return c % 32 < B2 ? gm[g.rDivM + ((c / 32) * B + r) * B2 + c % 32] : 0.0; }, B, 128,
-      S::MT, 4, S::rdmF, 7);
+      S::MT, 4, S::rdmF, 8);
   for (int m = 0; m < B2; ++m) tables[S::tInvM2 + m] = static_cast<float>(1.0 / gm[g.mass2 + m]);
   // fP_h rows as (value, column) lists of O entries, padded with zeros
   for (int h = 0; h < 3; ++h) {
