@@ -43,6 +43,17 @@ def algorithmic_bytes(order: int, real_bytes: int) -> dict:
     return {"local": local, "neigh": neigh, "update": local + neigh}
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), as the CPU-baseline plan asks."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(order: int, elements: int, steps: int, seed: int) -> dict:
     import oracle
     from seissol_amd.kernels import global_matrices, monomial_integrate
@@ -59,6 +70,7 @@ def cpu_baseline(order: int, elements: int, steps: int, seed: int) -> dict:
     sec = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": setup.n * steps / sec, "unit": "element-updates/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"ORDER {order}, {setup.n} elements x {steps} steps, proxy `all`, FP64 OpenMP oracle "
                       f"(CPU restatement, not the reference binary), {sec:.1f} s"}
 

@@ -34,6 +34,13 @@ extern "C" {
 #define SDG_F16X3 4  /* FP32 storage, MFMA with 2-plane fp16 splits of power-of-two scaled operands
                         (3 products), FP32 accumulate */
 
+/* storage of the per-cell star matrices (sdg_engine_set_layout) */
+#define SDG_STAR_DENSE 0   /* [3][81]: three 9x9 column-major matrices                          */
+#define SDG_STAR_COMPACT 1 /* [3][24]: the 24 non-zeros of each, column by column with rows
+                              ascending (the sparse memory layout of the reference's star tensor:
+                              codegen/matrices/star.xml, codegen/config/gpu/sparse.xml:26,
+                              LocalIntegrationData::starMatrices, src/Initializer/Typedefs.h:189) */
+
 /* face types (src/Initializer/CellLocalInformation.h FaceType subset) */
 #define SDG_FACE_REGULAR 0
 #define SDG_FACE_DYNAMIC_RUPTURE 1
@@ -96,11 +103,38 @@ int sdg_engine_create(sdg_engine** engine, int order, int arithmetic);
  * sdg_global_matrices). */
 int sdg_engine_set_global_matrices(sdg_engine* engine, const double* host_matrices);
 int sdg_engine_destroy(sdg_engine* engine);
-int64_t sdg_derivatives_size(int order); /* reals per element: sum_i 9*B(O-i) */
+int64_t sdg_derivatives_size(int order); /* reals per element, packed: sum_i 9*B(O-i) */
+
+/* Storage layout of the cells' arrays for every later call on this engine
+ * (the reference fixes it at build time):
+ *  alignment (bytes, 0 = packed, else a power of two that is a multiple of
+ *    sizeof(real), at most 1024): every DOF-like block (dofs, time_integrated /
+ *    LTS buffers, sdg_evaluate_batched's output) is [9][ld] per cell with
+ *    ld = B rounded up to a multiple of alignment / sizeof(real), the
+ *    quantities' padding rows never read or written -- tensor::Q and
+ *    tensor::I with alignStride (codegen/kernels/aderdg.py:58-65); and
+ *    derivative block i is [9][ld_i], ld_i = B(O-i) rounded up likewise, the
+ *    blocks one after another -- tensor::dQ(i) (the size check of the HIP
+ *    taylorSum, src/Kernels/LinearCK/DeviceAux/cudahip/KernelsAux.cpp:53-66).
+ *    The reference's AMD GPU build uses 128 (VECTORSIZE/ALIGNMENT,
+ *    cmake/process_users_input.cmake:402-404): ld = 64 at ORDER 6.
+ *  star_format: SDG_STAR_DENSE or SDG_STAR_COMPACT.
+ * The per-cell strides of star/aplus/aminus are fields of each sdg_layer. */
+int sdg_engine_set_layout(sdg_engine* engine, int alignment, int star_format);
+/* Sizes under the engine's layout: ld of a DOF-like block, reals per cell of
+ * a DOF-like block (9 ld) and of the derivatives. Any output may be NULL. */
+int sdg_layout_sizes(const sdg_engine* engine, int32_t* dofs_ld, int64_t* dofs_reals, int64_t* derivatives_reals);
 
 /* A layer of cells in device memory.  `real` is double for SDG_FP64 and
- * float otherwise.  DOF-like arrays are [n][9][B] (basis index fastest);
- * derivatives are [n][sdg_derivatives_size] with block i stored [9][B(O-i)]. */
+ * float otherwise.  DOF-like arrays are [n][9][ld] (basis index fastest,
+ * ld = B unless the engine's layout aligns it); derivatives are
+ * [n][derivatives_reals] with block i stored [9][ld_i] (sdg_engine_set_layout,
+ * sdg_layout_sizes).  The per-cell operators of cell e start at
+ * e * stride reals (0 = the dense default: star 243 or 72, aplus and aminus
+ * 324), so the fields of an array of structs such as the reference's
+ * LocalIntegrationData {starMatrices[3][24], nApNm1[4][81], ...} and
+ * NeighboringIntegrationData {nAmNm1[4][81], ...} (src/Initializer/Typedefs.h:186-203)
+ * are passed as they lie, with stride = sizeof(struct) / sizeof(real). */
 typedef struct {
   int64_t num_elements;
   void* dofs;                     /* [n][9][B]                                */
@@ -120,6 +154,9 @@ typedef struct {
                                   /* local pass, read by the neighbour pass)  */
   const uint16_t* lts_setup;      /* [n] LTS bits (SDG_LTS_*) or NULL: global */
                                   /* time stepping                            */
+  int64_t star_stride;            /* reals between cells' star matrices, 0: dense */
+  int64_t aplus_stride;           /* reals between cells' AplusT blocks, 0: 324 */
+  int64_t aminus_stride;          /* reals between cells' AminusT blocks, 0: 324 */
 } sdg_layer;
 
 /* Local time stepping bits of a cell (src/Initializer/LtsSetup.h:18-110). */

@@ -20,6 +20,8 @@ SDG_F16X3 = 4
 SDG_LTS_HAS_BUFFERS = 1 << 8
 SDG_LTS_HAS_DERIVATIVES = 1 << 9
 SDG_LTS_ACCUMULATE_BUFFERS = 1 << 10
+SDG_STAR_DENSE = 0
+SDG_STAR_COMPACT = 1
 SDG_FACE_REGULAR = 0
 SDG_FACE_DYNAMIC_RUPTURE = 1
 
@@ -47,6 +49,9 @@ class Layer(C.Structure):
         ("face_types", P),
         ("face_traces", P),
         ("lts_setup", P),
+        ("star_stride", I64),
+        ("aplus_stride", I64),
+        ("aminus_stride", I64),
     ]
 
 
@@ -124,6 +129,8 @@ SIGNATURES = {
     "sdg_engine_set_global_matrices": (I32, [P, P]),
     "sdg_engine_destroy": (I32, [P]),
     "sdg_derivatives_size": (I64, [I32]),
+    "sdg_engine_set_layout": (I32, [P, I32, I32]),
+    "sdg_layout_sizes": (I32, [P, P, P, P]),
     "sdg_compute_batched_ader": (I32, [P, P, D, C.POINTER(Layer), P]),
     "sdg_evaluate_batched": (I32, [P, P, P, P, I64, P]),
     "sdg_compute_batched_integral": (I32, [P, C.POINTER(Layer), P]),

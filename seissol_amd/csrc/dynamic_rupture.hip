@@ -79,6 +79,22 @@ __device__ inline void waveSync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// position of entry i of the packed derivatives (blocks [9][B(O-d)] one
+// after another) in the layout's derivatives (blocks [9][rows(B(O-d))])
+template <int O>
+__device__ inline int derivativeOffset(int i, const CellLayout& lay) {
+  if (lay.align == 1) return i;
+  int rem = i, off = 0, at = 0;
+#pragma unroll
+  for (int d = 0; d < O; ++d) {
+    const int m = numBasis3(O - d);
+    if (rem >= 0 && rem < 9 * m) at = off + (rem / m) * lay.rows(m) + rem % m;
+    rem -= 9 * m;
+    off += 9 * lay.rows(m);
+  }
+  return at;
+}
+
 // Space-time interpolation, batched by the 16 variants of V3mTo2n(i, h)
 // (h = 0: plus side, h = 1..3: minus side with that orientation), as the
 // reference batches per (side, faceRelation): workgroup (chunk, variant)
@@ -90,7 +106,7 @@ __device__ inline void waveSync() {
 // q_interpolated layout; points k in [(O+1)^2, P) are written as zero).
 template <typename real, int O>
 __global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, const real* __restrict__ V,
-                                                                const real* __restrict__ deriv, int64_t dsize,
+                                                                const real* __restrict__ deriv, CellLayout lay,
                                                                 sdg_dr_layer L, StepCoeffs<O> cf,
                                                                 double* __restrict__ outPlus,
                                                                 double* __restrict__ outMinus) {
@@ -115,7 +131,9 @@ __global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, c
     int elem = -1;
     if (f < nf) {
       const int faceSide = min(static_cast<int>(L.face_info[f * 3 + side]), 3);
-      const int h = side == 0 ? 0 : min(static_cast<int>(L.face_info[f * 3 + 2]), 3);
+      // faceRelation = orientation + 1 lies in [1, 3]; clamped so that a malformed
+      // entry still names a minus-side variant (0 would match no workgroup)
+      const int h = side == 0 ? 0 : min(max(static_cast<int>(L.face_info[f * 3 + 2]), 1), 3);
       if (faceSide == vi && h == vh) elem = side == 0 ? L.plus_element[f] : L.minus_element[f];
     }
     sElem[threadIdx.x] = elem;
@@ -138,8 +156,8 @@ __global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, c
       real tmp[6];
 #pragma unroll
       for (int u = 0; u < 6; ++u) {
-        const int i = lane + 64 * (c + u);
-        tmp[u] = deriv[e * dsize + (i < DS ? i : DS - 1)];
+        const int i = min(lane + 64 * (c + u), DS - 1);
+        tmp[u] = deriv[e * lay.dsize + derivativeOffset<O>(i, lay)];
       }
 #pragma unroll
       for (int u = 0; u < 6; ++u) {
@@ -205,7 +223,7 @@ __global__ __launch_bounds__(kDrThreads) void kDrNodalFlux(int64_t n, const real
                                                            const int32_t* __restrict__ drMap, sdg_dr_layer L,
                                                            const double* __restrict__ impPlus,
                                                            const double* __restrict__ impMinus,
-                                                           real* __restrict__ dofs) {
+                                                           real* __restrict__ dofs, CellLayout lay) {
   constexpr int B = numBasis3(O), NP = (O + 1) * (O + 1);
   __shared__ real sGF[64 * 9];
   __shared__ real sF[81];
@@ -229,7 +247,7 @@ __global__ __launch_bounds__(kDrThreads) void kDrNodalFlux(int64_t n, const real
     const int plusSide = __builtin_amdgcn_readfirstlane(static_cast<int>(L.face_info[m * 3 + 0]));
     const int relation = __builtin_amdgcn_readfirstlane(static_cast<int>(L.face_info[m * 3 + 2]));
     const bool plus = plusElem == e && plusSide == face;
-    const int h = plus ? 0 : min(relation, 3);
+    const int h = plus ? 0 : min(max(relation, 1), 3);  // minus side: faceRelation in [1, 3]
     const double* G = (plus ? impPlus : impMinus) + static_cast<int64_t>(m) * 9 * P;
     const double* F = (plus ? L.flux_solver_plus : L.flux_solver_minus) + static_cast<int64_t>(m) * 81;
     __syncthreads();
@@ -262,7 +280,7 @@ __global__ __launch_bounds__(kDrThreads) void kDrNodalFlux(int64_t n, const real
     const int k = tid + r * kDrThreads;
     if (k < B)
 #pragma unroll
-      for (int p = 0; p < 9; ++p) dofs[(e * 9 + p) * B + k] += acc[r][p];
+      for (int p = 0; p < 9; ++p) dofs[e * lay.qStride + p * lay.ldq + k] += acc[r][p];
   }
 }
 
@@ -273,11 +291,10 @@ int interpolateImpl(EngineState& st, const double* coeffs, const void* deriv, co
   StepCoeffs<O> cf{};
   for (int o = 0; o < O; ++o)
     for (int d = 0; d < O; ++d) cf.c[o][d] = coeffs[o * O + d];
-  int64_t dsize = 0;
-  for (int i = 0; i < O; ++i) dsize += 9LL * numBasis3(O - i);
   const dim3 grid(static_cast<unsigned>((L->num_faces + kDrChunk - 1) / kDrChunk), 16);
   kDrInterpolate<real, O><<<grid, 64 * kDrWaves, 0, stream>>>(L->num_faces, static_cast<const real*>(st.drMatrices),
-                                                              static_cast<const real*>(deriv), dsize, *L, cf, qp, qm);
+                                                              static_cast<const real*>(deriv), engineLayout(st), *L,
+                                                              cf, qp, qm);
   SDG_HIP_CHECK(hipGetLastError());
   return SDG_OK;
 }
@@ -289,7 +306,8 @@ int nodalFluxImpl(EngineState& st, const sdg_layer* layer, const int32_t* drMap,
   if (n == 0 || L->num_faces == 0) return SDG_OK;
   const int64_t v3mSize = 16LL * stroudPoints(O) * numBasis3(O);
   kDrNodalFlux<real, O><<<static_cast<unsigned>(n), kDrThreads, 0, stream>>>(
-      n, static_cast<const real*>(st.drMatrices) + v3mSize, drMap, *L, ip, im, static_cast<real*>(layer->dofs));
+      n, static_cast<const real*>(st.drMatrices) + v3mSize, drMap, *L, ip, im, static_cast<real*>(layer->dofs),
+      cellLayout(st, layer));
   SDG_HIP_CHECK(hipGetLastError());
   return SDG_OK;
 }
@@ -309,6 +327,12 @@ extern "C" int sdg_dr_face_data(int64_t num_faces, const double* vertices, const
     setError("sdg_dr_face_data: invalid arguments");
     return SDG_ERR_INVALID;
   }
+  for (int64_t m = 0; m < num_faces; ++m)
+    if (face_info[m * 3] > 3 || face_info[m * 3 + 1] > 3 || face_info[m * 3 + 2] < 1 || face_info[m * 3 + 2] > 3) {
+      setError("sdg_dr_face_data: face " + std::to_string(m) +
+               ": sides must lie in [0, 3] and faceRelation (orientation + 1) in [1, 3]");
+      return SDG_ERR_INVALID;
+    }
   for (int64_t m = 0; m < num_faces; ++m) {
     const int64_t ep = plus_element[m], em = minus_element[m];
     const int sp = face_info[m * 3] & 3;

@@ -38,9 +38,29 @@ inline bool matrixCore(int arithmetic) {
   return arithmetic == SDG_BF16X3 || arithmetic == SDG_BF16X6 || arithmetic == SDG_F16X3;
 }
 
+// Storage layout of a layer's per-cell arrays as the kernels see them.
+// DOF-like blocks (dofs, buffers, Taylor outputs) are [9][ldq] per cell, a
+// derivative block of m rows is [9][rows(m)], the blocks one after another;
+// the per-cell operators sit starStride / aplusStride / aminusStride reals
+// apart, the star matrices as 3 x 81 dense or 3 x 24 non-zeros.
+struct CellLayout {
+  int32_t ldq = 0;
+  int32_t align = 1;      // rows of a block are rounded up to a multiple of this (a power of two)
+  int64_t qStride = 0;    // 9 * ldq
+  int64_t dsize = 0;      // reals of one cell's derivatives
+  int64_t starStride = 0, aplusStride = 324, aminusStride = 324;
+  int32_t starCompact = 0;
+  int32_t pad_ = 0;
+  __host__ __device__ int rows(int m) const { return (m + align - 1) & ~(align - 1); }
+};
+
 struct EngineState {
   int order = 0;
   int arithmetic = SDG_FP64;
+  // layout of the cells' arrays (sdg_engine_set_layout): alignment in bytes
+  // (0 = packed) and the star format
+  int alignment = 0;
+  int starFormat = SDG_STAR_DENSE;
   void* matrices = nullptr;  // device, real type of the engine
   DeviceMatrixLayout layout{};
   // matrix-core arithmetics: pre-split 16-bit fragments and FP32 tables
@@ -49,6 +69,13 @@ struct EngineState {
   // dynamic rupture face matrices (V3mTo2n, V3mTo2nTWDivM), on first use
   void* drMatrices = nullptr;
 };
+
+// the layout of `layer` (its strides, 0 = the dense defaults) under the engine's settings
+CellLayout cellLayout(const EngineState& e, const sdg_layer* layer);
+// the same for arrays handed over without a layer (derivatives, Taylor outputs)
+CellLayout engineLayout(const EngineState& e);
+// checks the per-cell strides of `layer` against the minimum sizes; sets the error text
+bool layerStridesOk(const EngineState& e, const sdg_layer* layer, const char* who);
 
 }  // namespace sdg
 

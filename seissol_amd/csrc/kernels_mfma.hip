@@ -32,6 +32,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <string>
+#include <type_traits>
 
 #include "engine.h"
 #include "mfma_common.h"
@@ -294,14 +296,20 @@ __device__ inline void gemmTile(floatx4 (&acc)[JJ][NQ], const uint16_t* const (&
 }
 
 // The lane's four consecutive basis rows row0..row0+3 of the 9 quantities of
-// its element at q[p * B + j] (row0 = 16 w + 4 g): one 16-byte access per
-// quantity when B is a multiple of 4 (rows 16-byte aligned), else per value.
-template <int B>
-__device__ inline void loadRows(const float* q, int row0, bool valid, float (&v)[9][4]) {
-  if constexpr (B % 4 == 0) {
+// its element at q[p * ld + j] (row0 = 16 w + 4 g): one 16-byte access per
+// quantity when V4 (ld a multiple of 4, the block 16-byte aligned), else per
+// value; rows from B on (the padding of an aligned layout) are never read.
+template <int B, bool V4>
+__device__ inline void loadRows(const float* q, int ld, int row0, bool valid, float (&v)[9][4]) {
+  if constexpr (V4) {
 #pragma unroll
     for (int p = 0; p < 9; ++p) {
-      const float4 x = (valid && row0 < B) ? *reinterpret_cast<const float4*>(q + p * B) : float4{0, 0, 0, 0};
+      float4 x = (valid && row0 < B) ? *reinterpret_cast<const float4*>(q + p * ld) : float4{0, 0, 0, 0};
+      if constexpr (B % 4 != 0) {
+        if (row0 + 1 >= B) x.y = 0.0f;
+        if (row0 + 2 >= B) x.z = 0.0f;
+        if (row0 + 3 >= B) x.w = 0.0f;
+      }
       v[p][0] = x.x;
       v[p][1] = x.y;
       v[p][2] = x.z;
@@ -311,19 +319,20 @@ __device__ inline void loadRows(const float* q, int row0, bool valid, float (&v)
 #pragma unroll
     for (int p = 0; p < 9; ++p)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) v[p][j] = (valid && row0 + j < B) ? q[p * B + j] : 0.0f;
+      for (int j = 0; j < 4; ++j) v[p][j] = (valid && row0 + j < B) ? q[p * ld + j] : 0.0f;
   }
 }
 
-// q[p * B + j] += (add ? add[p][j] : 0) + acc[p][j] * s * un[p] for the lane's rows
-template <int B>
-__device__ inline void updateRows(float* q, int row0, const floatx4 (&acc)[9], float s, const float (&un)[9],
+// q[p * ld + j] += (add ? add[p][j] : 0) + acc[p][j] * s * un[p] for the
+// lane's rows below B (16-byte accesses when V4 and the four rows are all real)
+template <int B, bool V4>
+__device__ inline void updateRows(float* q, int ld, int row0, const floatx4 (&acc)[9], float s, const float (&un)[9],
                                   const float (*add)[4]) {
-  if constexpr (B % 4 == 0) {
+  if (V4 && (B % 4 == 0 || row0 + 3 < B)) {
     if (row0 < B)
 #pragma unroll
       for (int p = 0; p < 9; ++p) {
-        float4* d = reinterpret_cast<float4*>(q + p * B);
+        float4* d = reinterpret_cast<float4*>(q + p * ld);
         float4 x = *d;
         const float sc = s * un[p];
         x.x += (add ? add[p][0] : 0.0f) + acc[p][0] * sc;
@@ -337,7 +346,7 @@ __device__ inline void updateRows(float* q, int row0, const floatx4 (&acc)[9], f
     for (int p = 0; p < 9; ++p)
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (row0 + j < B) q[p * B + j] += (add ? add[p][j] : 0.0f) + acc[p][j] * (s * un[p]);
+        if (row0 + j < B) q[p * ld + j] += (add ? add[p][j] : 0.0f) + acc[p][j] * (s * un[p]);
   }
 }
 
@@ -399,11 +408,12 @@ __device__ inline void gemmStars(const uint16_t* const (&a)[3], int ks, const ty
   }
 }
 
-template <int O, int V>
+template <int O, int V, bool V4>
 __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLocalMfma(
     int64_t n, const uint16_t* __restrict__ frag, const float* __restrict__ tables, CoeffsF<O> cf,
     float* __restrict__ dofs, float* __restrict__ integ, float* __restrict__ traces,
-    const float* __restrict__ star, const float* __restrict__ aplus, const uint8_t* __restrict__ faceTypes) {
+    const float* __restrict__ star, const float* __restrict__ aplus, const uint8_t* __restrict__ faceTypes,
+    CellLayout lay) {
   using S = MShape<O>;
   using C = Cfg<O, V>;
   using T = typename Prec<V>::T;
@@ -423,8 +433,9 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
   const bool valid = elem < n;
   // per-lane addresses as 32-bit offsets from uniform tile bases (one SGPR
   // pair each) rather than 36 live 64-bit addresses
-  float* const tileDofs = dofs + E0 * 9 * B;
-  const int laneQ = e * 9 * B + 16 * w + 4 * g;
+  const int ldq = lay.ldq;
+  float* const tileDofs = dofs + E0 * lay.qStride;
+  const int laneQ = e * static_cast<int>(lay.qStride) + 16 * w + 4 * g;
   float* const tileTraces = traces + E0 * 36 * S::B2P;
   const float* aUn = tables + S::tAUn;
   const float aUnCk[3] = {aUn[0], aUn[1], aUn[2]};
@@ -441,7 +452,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
       const int i = threadIdx.x + u * S::THREADS;
       const int s = min(i / 72, kTileElems - 1), r = i % 72;
       const int64_t el = E0 + s < n ? E0 + s : n - 1;
-      sv[u] = star[el * 243 + 81 * (r / 24) + kStarPos[r % 24]];
+      sv[u] = star[el * lay.starStride + (lay.starCompact ? r : 81 * (r / 24) + kStarPos[r % 24])];
     }
 #pragma unroll
     for (int u = 0; u < SI; ++u) {
@@ -452,7 +463,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
 
   // Q itself is read again at the end, so only the time integral stays live
   float q[1][9][4], iacc[1][9][4];
-  loadRows<B>(tileDofs + laneQ, 16 * w + 4 * g, valid, q[0]);
+  loadRows<B, V4>(tileDofs + laneQ, ldq, 16 * w + 4 * g, valid, q[0]);
 #pragma unroll
   for (int p = 0; p < 9; ++p)
 #pragma unroll
@@ -553,13 +564,13 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
 
   // time-integrated DOFs: optional output, and the operand of the next GEMMs
   if (integ != nullptr && valid) {
-    float* const tileInteg = integ + E0 * 9 * B;
+    float* const tileInteg = integ + E0 * lay.qStride;
 #pragma unroll
     for (int p = 0; p < 9; ++p)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int row = 16 * w + 4 * g + j;
-        if (row < B) tileInteg[laneQ + p * B + j] = iacc[0][p][j];
+        if (row < B) tileInteg[laneQ + p * ldq + j] = iacc[0][p][j];
       }
   }
   if constexpr (scaled) {
@@ -637,7 +648,8 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
 #pragma unroll
   for (int i = 0; i < APT; ++i) {
     const int idx = threadIdx.x + i * S::THREADS;
-    apv[i] = (idx < kTileElems * 324 && E0 + idx / 324 < n) ? aplus[E0 * 324 + idx] : 0.0f;
+    apv[i] = (idx < kTileElems * 324 && E0 + idx / 324 < n) ? aplus[(E0 + idx / 324) * lay.aplusStride + idx % 324]
+                                                             : 0.0f;
   }
   __syncthreads();
 #pragma unroll
@@ -711,15 +723,15 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     }
     const uint16_t* a[1] = {frag + S::rdm + static_cast<int64_t>(w) * S::KSF * kFragPair};
     gemmTile<V, 1, 0, 9, LDX>(acc, a, S::KSF, xs, lane);
-    if (valid) updateRows<B>(tileDofs + laneQ, 16 * w + 4 * g, acc[0], aUnR, un, nullptr);
+    if (valid) updateRows<B, V4>(tileDofs + laneQ, ldq, 16 * w + 4 * g, acc[0], aUnR, un, nullptr);
   }
 }
 
-template <int O, int V>
+template <int O, int V, bool V4>
 __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNeighborMfma(
     int64_t n, const uint16_t* __restrict__ frag, const float* __restrict__ tables, float* __restrict__ dofs,
     const float* __restrict__ traces, const int32_t* __restrict__ neighbors, const uint8_t* __restrict__ faceRel,
-    const float* __restrict__ aminus, const uint8_t* __restrict__ faceTypes) {
+    const float* __restrict__ aminus, const uint8_t* __restrict__ faceTypes, CellLayout lay) {
   using S = MShape<O>;
   using C = Cfg<O, V>;
   using T = typename Prec<V>::T;
@@ -747,8 +759,8 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   const int g = lane >> 4, e = lane & 15;
   const int64_t E0 = static_cast<int64_t>(blockIdx.x) * kTileElems;
   const bool valid = E0 + e < n;
-  float* const tileDofs = dofs + E0 * 9 * B;
-  const int laneQ = e * 9 * B + 16 * w + 4 * g;
+  float* const tileDofs = dofs + E0 * lay.qStride;
+  const int laneQ = e * static_cast<int>(lay.qStride) + 16 * w + 4 * g;
 
   for (int i = threadIdx.x; i < 3 * B2 * O; i += blockDim.x) {
     sFP[i] = tables[S::tFP + i];
@@ -793,8 +805,9 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
 #pragma unroll
     for (int i = 0; i < APT; ++i) {
       const int idx = threadIdx.x + i * S::THREADS;
-      av[i] = (idx < kTileElems * 81 && E0 + idx / 81 < n) ? aminus[E0 * 324 + (idx / 81) * 324 + f * 81 + idx % 81]
-                                                            : 0.0f;
+      av[i] = (idx < kTileElems * 81 && E0 + idx / 81 < n)
+                  ? aminus[(E0 + idx / 81) * lay.aminusStride + f * 81 + idx % 81]
+                  : 0.0f;
     }
   };
   gather(0);
@@ -899,7 +912,20 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
 #pragma unroll
   for (int p = 0; p < 9; ++p) one[p] = 1.0f;
   const float aUnR = tables[S::tAUn + 8];
-  if (valid) updateRows<B>(tileDofs + laneQ, 16 * w + 4 * g, tot[0], aUnR, one, nullptr);
+  if (valid) updateRows<B, V4>(tileDofs + laneQ, lay.ldq, 16 * w + 4 * g, tot[0], aUnR, one, nullptr);
+}
+
+// 16-byte DOF accesses: every block starts on a multiple of 4 reals and the
+// arrays are 16-byte aligned (always so for B a multiple of 4 on aligned arrays)
+inline bool vectorAccess(const CellLayout& lay, const void* dofs, const void* integ, const char* who, bool& v4) {
+  const bool aligned = (reinterpret_cast<uintptr_t>(dofs) % 16 == 0) &&
+                       (integ == nullptr || reinterpret_cast<uintptr_t>(integ) % 16 == 0);
+  v4 = lay.ldq % 4 == 0 && aligned;
+  if (lay.ldq % 4 == 0 && !aligned) {
+    setError(std::string(who) + ": dofs and time_integrated must be 16-byte aligned");
+    return false;
+  }
+  return true;
 }
 
 template <int O, int V>
@@ -910,12 +936,23 @@ int localImpl(const EngineState& st, const double* coeffs, const sdg_layer* L, b
   if (n == 0) return SDG_OK;
   CoeffsF<O> cf{};
   for (int i = 0; i < O; ++i) cf.c[i] = static_cast<float>(coeffs[i]);
+  const CellLayout lay = cellLayout(st, L);
+  float* integ = writeIntegrated ? static_cast<float*>(L->time_integrated) : nullptr;
+  bool v4 = false;
+  if (!vectorAccess(lay, L->dofs, integ, "matrix-core local pass", v4)) return SDG_ERR_INVALID;
   const dim3 grid(static_cast<unsigned>((n + kTileElems - 1) / kTileElems));
-  kLocalMfma<O, V><<<grid, S::THREADS, 0, stream>>>(
-      n, static_cast<const uint16_t*>(st.mfmaFrags), static_cast<const float*>(st.mfmaTables), cf,
-      static_cast<float*>(L->dofs), writeIntegrated ? static_cast<float*>(L->time_integrated) : nullptr,
-      static_cast<float*>(L->face_traces), static_cast<const float*>(L->star), static_cast<const float*>(L->aplus),
-      L->face_types);
+  auto launch = [&](auto vec) {
+    kLocalMfma<O, V, decltype(vec)::value><<<grid, S::THREADS, 0, stream>>>(
+        n, static_cast<const uint16_t*>(st.mfmaFrags), static_cast<const float*>(st.mfmaTables), cf,
+        static_cast<float*>(L->dofs), integ, static_cast<float*>(L->face_traces), static_cast<const float*>(L->star),
+        static_cast<const float*>(L->aplus), L->face_types, lay);
+  };
+  if constexpr (S::B % 4 != 0) {
+    if (!v4) launch(std::false_type{});
+    else launch(std::true_type{});
+  } else {
+    launch(std::true_type{});
+  }
   SDG_HIP_CHECK(hipGetLastError());
   return SDG_OK;
 }
@@ -925,11 +962,22 @@ int neighborImpl(const EngineState& st, const sdg_layer* L, hipStream_t stream) 
   using S = MShape<O>;
   const int64_t n = L->num_elements;
   if (n == 0) return SDG_OK;
+  const CellLayout lay = cellLayout(st, L);
+  bool v4 = false;
+  if (!vectorAccess(lay, L->dofs, nullptr, "matrix-core neighbour pass", v4)) return SDG_ERR_INVALID;
   const dim3 grid(static_cast<unsigned>((n + kTileElems - 1) / kTileElems));
-  kNeighborMfma<O, V><<<grid, S::THREADS, 0, stream>>>(
-      n, static_cast<const uint16_t*>(st.mfmaFrags), static_cast<const float*>(st.mfmaTables),
-      static_cast<float*>(L->dofs), static_cast<const float*>(L->face_traces), L->face_neighbors, L->face_relations,
-      static_cast<const float*>(L->aminus), L->face_types);
+  auto launch = [&](auto vec) {
+    kNeighborMfma<O, V, decltype(vec)::value><<<grid, S::THREADS, 0, stream>>>(
+        n, static_cast<const uint16_t*>(st.mfmaFrags), static_cast<const float*>(st.mfmaTables),
+        static_cast<float*>(L->dofs), static_cast<const float*>(L->face_traces), L->face_neighbors,
+        L->face_relations, static_cast<const float*>(L->aminus), L->face_types, lay);
+  };
+  if constexpr (S::B % 4 != 0) {
+    if (!v4) launch(std::false_type{});
+    else launch(std::true_type{});
+  } else {
+    launch(std::true_type{});
+  }
   SDG_HIP_CHECK(hipGetLastError());
   return SDG_OK;
 }

@@ -74,20 +74,27 @@ struct LtsArgs {
 template <int O, typename real>
 struct NeighborLts {
   const uint16_t* setup;  // [n] LTS bits or nullptr: every neighbour provides a buffer
-  const real* deriv;      // derivatives of all cells, [*][dsize], indexed like the buffers
-  int64_t dsize;
+  const real* deriv;      // derivatives of all cells (the layer's layout), indexed like the buffers
   Coeffs<O, real> gts, sub;  // time-basis coefficients: GTS relation, sub-interval of a larger step
 };
 
 // Gather the 24 non-zeros of the three star matrices of the elements of the
-// block into LDS (compact order: p-major, pattern order).
+// block into LDS (compact order: output quantity p major, then the input
+// quantities of its row of the pattern, the reference's sparse order), from
+// dense [3][81] or compact [3][24] storage, cells starStride reals apart.
 template <typename real>
-__device__ inline void loadStars(real* sStar, const real* __restrict__ star, int64_t e0, int64_t n, int epb) {
+__device__ inline void loadStars(real* sStar, const real* __restrict__ star, int64_t e0, int64_t n, int epb,
+                                 const CellLayout& lay) {
   for (int idx = threadIdx.x; idx < epb * 3 * kStarNnz; idx += blockDim.x) {
     const int s = idx / (3 * kStarNnz);
     const int w = idx % (3 * kStarNnz);
     const int j = w / kStarNnz;
     int z = w % kStarNnz;
+    const int64_t e = e0 + s;
+    if (lay.starCompact) {
+      sStar[idx] = (e < n) ? star[e * lay.starStride + w] : real(0);
+      continue;
+    }
     int p = 0, k = 0;
     // position z -> (p, k)
     for (p = 0; p < 9; ++p) {
@@ -96,8 +103,7 @@ __device__ inline void loadStars(real* sStar, const real* __restrict__ star, int
       z -= cnt;
     }
     const int q = kStarQ[p][k];
-    const int64_t e = e0 + s;
-    sStar[idx] = (e < n) ? star[e * 243 + 81 * j + q + 9 * p] : real(0);
+    sStar[idx] = (e < n) ? star[e * lay.starStride + 81 * j + q + 9 * p] : real(0);
   }
 }
 
@@ -247,22 +253,25 @@ constexpr int productMode() {
   }
 }
 
-// a tile's [EPB][9][B] block of src (elements e0, e0 + 1, ...) into LDS as
-// [element][row][quantity], zeros past element n - 1; the loads go in chunks
-// of six per thread, each chunk issued before its first value is stored
-// (clamped addresses, in bounds: a launched tile has e0 < n)
+// a tile's DOF blocks of src (elements e0, e0 + 1, ..., each [9][ldq], the
+// B rows of each quantity read) into LDS as [element][row][quantity], zeros
+// past element n - 1; the loads go in chunks of six per thread, each chunk
+// issued before its first value is stored (clamped addresses, in bounds: a
+// launched tile has e0 < n)
 template <int EPB, int QS, int B, int THREADS, typename real>
-__device__ inline void stageBlock(real* sA, const real* __restrict__ src, int64_t e0, int64_t n) {
+__device__ inline void stageBlock(real* sA, const real* __restrict__ src, int64_t e0, int64_t n, int ldq,
+                                  int64_t qStride) {
   constexpr int TOT = EPB * QS;
   constexpr int NI = (TOT + THREADS - 1) / THREADS;
-  const int64_t last = n * QS - 1;
 #pragma unroll
   for (int c = 0; c < NI; c += 6) {
     real v[6];
 #pragma unroll
     for (int u = 0; u < 6; ++u) {
-      const int64_t g = e0 * QS + threadIdx.x + (c + u) * THREADS;
-      v[u] = src[g < last ? g : last];
+      const int idx = min(static_cast<int>(threadIdx.x) + (c + u) * THREADS, TOT - 1);
+      const int s = idx / QS, w = idx % QS;
+      const int64_t el = e0 + s < n ? e0 + s : n - 1;
+      v[u] = src[el * qStride + (w / B) * ldq + w % B];
     }
 #pragma unroll
     for (int u = 0; u < 6; ++u) {
@@ -281,7 +290,7 @@ __device__ inline void stageBlock(real* sA, const real* __restrict__ src, int64_
 template <typename real, int O, bool WithLocal, int RX = 0, int WV = 4>
 __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restrict__ mats, Coeffs<O, real> cf,
                                              real* __restrict__ dofs, real* __restrict__ integ,
-                                             real* __restrict__ deriv, int64_t dsize,
+                                             real* __restrict__ deriv, CellLayout lay,
                                              const real* __restrict__ star, const real* __restrict__ aplus,
                                              const uint8_t* __restrict__ faceTypes, LtsArgs lts) {
   using S = Shape<O, RX, WV>;
@@ -308,8 +317,8 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
   const bool accumulate = lts.setup != nullptr && !lts.reset && (setup & SDG_LTS_ACCUMULATE_BUFFERS);
 
   // stage Q as [row][quantity]
-  stageBlock<S::EPB, S::QS, B, S::THREADS>(sA, dofs, e0, n);
-  loadStars(sStar, star, e0, n, S::EPB);
+  stageBlock<S::EPB, S::QS, B, S::THREADS>(sA, dofs, e0, n, lay.ldq, lay.qStride);
+  loadStars(sStar, star, e0, n, S::EPB, lay);
   __syncthreads();
 
   const int sl = laneOk ? slot : 0;
@@ -332,11 +341,11 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
     for (int r = 0; r < R; ++r) {
       const int row = t + r * TPE;
       if (row < B)
-        for (int p = 0; p < 9; ++p) deriv[e * dsize + p * B + row] = qreg[r][p];
+        for (int p = 0; p < 9; ++p) deriv[e * lay.dsize + p * lay.ldq + row] = qreg[r][p];
     }
   }
 
-  int64_t doff = 9 * B;
+  int64_t doff = 9LL * lay.ldq;
   int nIn = B;
 #pragma unroll 1
   for (int d = 1; d < O; ++d) {
@@ -364,11 +373,11 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
           iacc[r][p] += cf.c[d] * dq[r][p];
         }
         if (storeDeriv)
-          for (int p = 0; p < 9; ++p) deriv[e * dsize + doff + p * m + row] = dq[r][p];
+          for (int p = 0; p < 9; ++p) deriv[e * lay.dsize + doff + p * lay.rows(m) + row] = dq[r][p];
       }
     }
     __syncthreads();
-    doff += 9 * m;
+    doff += 9LL * lay.rows(m);
     nIn = m;
   }
 
@@ -379,7 +388,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
       if (row < B)
 #pragma unroll
         for (int p = 0; p < 9; ++p) {
-          real* b = integ + e * S::QS + p * B + row;
+          real* b = integ + e * lay.qStride + p * lay.ldq + row;
           *b = accumulate ? *b + iacc[r][p] : iacc[r][p];
         }
     }
@@ -421,7 +430,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
         for (int p = 0; p < 9; ++p) tr[0][r][p] = real(0);
       gemmRows<1, R, kProd>(tr, mats + S::fmrt + static_cast<int64_t>(f) * B * B2, 0, B2, xa, B, t, TPE, B2);
       const bool skip = !active || (faceTypes != nullptr && faceTypes[e * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE);
-      const real* A = aplus + (active ? e : 0) * 324 + 81 * f;
+      const real* A = aplus + (active ? e : 0) * lay.aplusStride + 81 * f;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int row = t + r * TPE;
@@ -451,7 +460,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
           const int row = t + r * TPE;
           if (row < B)
 #pragma unroll
-            for (int p = 0; p < 9; ++p) dofs[e * S::QS + p * B + row] = acc[0][r][p];
+            for (int p = 0; p < 9; ++p) dofs[e * lay.qStride + p * lay.ldq + row] = acc[0][r][p];
         }
       }
     }
@@ -465,7 +474,7 @@ template <typename real, int O>
 __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict__ mats, real* __restrict__ dofs,
                                               const real* __restrict__ integ, const real* __restrict__ star,
                                               const real* __restrict__ aplus,
-                                              const uint8_t* __restrict__ faceTypes) {
+                                              const uint8_t* __restrict__ faceTypes, CellLayout lay) {
   using S = Shape<O>;
   constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
   constexpr int kProd = productMode<real, O, 0, false>();
@@ -480,8 +489,8 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
   const bool laneOk = lane < S::EPW * TPE;
   const int64_t e = e0 + slot;
   const bool active = laneOk && e < n;
-  stageBlock<S::EPB, S::QS, B, S::THREADS>(sA, integ, e0, n);
-  loadStars(sStar, star, e0, n, S::EPB);
+  stageBlock<S::EPB, S::QS, B, S::THREADS>(sA, integ, e0, n, lay.ldq, lay.qStride);
+  loadStars(sStar, star, e0, n, S::EPB, lay);
   __syncthreads();
   const int sl = laneOk ? slot : 0;
   real* xa = sA + sl * S::QS;
@@ -493,7 +502,7 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
   for (int r = 0; r < R; ++r) {
     const int row = t + r * TPE;
 #pragma unroll
-    for (int p = 0; p < 9; ++p) qreg[r][p] = (active && row < B) ? dofs[e * S::QS + p * B + row] : real(0);
+    for (int p = 0; p < 9; ++p) qreg[r][p] = (active && row < B) ? dofs[e * lay.qStride + p * lay.ldq + row] : real(0);
   }
   {
     real acc[3][R][9];
@@ -520,7 +529,7 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
       for (int p = 0; p < 9; ++p) tr[0][r][p] = real(0);
     gemmRows<1, R, kProd>(tr, mats + S::fmrt + static_cast<int64_t>(f) * B * B2, 0, B2, xa, B, t, TPE, B2);
     const bool skip = !active || (faceTypes != nullptr && faceTypes[e * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE);
-    const real* A = aplus + (active ? e : 0) * 324 + 81 * f;
+    const real* A = aplus + (active ? e : 0) * lay.aplusStride + 81 * f;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int row = t + r * TPE;
@@ -549,7 +558,7 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
       const int row = t + r * TPE;
       if (row < B)
 #pragma unroll
-        for (int p = 0; p < 9; ++p) dofs[e * S::QS + p * B + row] = acc[0][r][p];
+        for (int p = 0; p < 9; ++p) dofs[e * lay.qStride + p * lay.ldq + row] = acc[0][r][p];
     }
   }
 }
@@ -563,7 +572,8 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
                                                  const int32_t* __restrict__ neighbors,
                                                  const uint8_t* __restrict__ faceRel,
                                                  const real* __restrict__ aminus,
-                                                 const uint8_t* __restrict__ faceTypes, NeighborLts<O, real> lts) {
+                                                 const uint8_t* __restrict__ faceTypes, NeighborLts<O, real> lts,
+                                                 CellLayout lay) {
   using S = Shape<O, RX, WV>;
   constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
   constexpr int kProd = productMode<real, O, RX, true>();
@@ -606,8 +616,8 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
         real gv[6];
 #pragma unroll
         for (int u = 0; u < 6; ++u) {
-          const int idx = t + (c + u) * TPE;
-          gv[u] = integ[nb * S::QS + (idx < S::QS ? idx : S::QS - 1)];
+          const int idx = min(t + (c + u) * TPE, S::QS - 1);
+          gv[u] = integ[nb * lay.qStride + (idx / B) * lay.ldq + idx % B];
         }
 #pragma unroll
         for (int u = 0; u < 6; ++u) {
@@ -622,12 +632,12 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
       for (int idx = t; idx < S::QS; idx += TPE) {
         const int p = idx / B, l = idx % B;
         real dv[O];
-        int64_t off = nb * lts.dsize;
+        int64_t off = nb * lay.dsize;
 #pragma unroll
         for (int i = 0; i < O; ++i) {
-          const int m = basis3(O - i);
-          dv[i] = lts.deriv[off + (l < m ? p * m + l : 0)];
-          off += 9 * m;
+          const int m = basis3(O - i), ld = lay.rows(m);
+          dv[i] = lts.deriv[off + (l < m ? p * ld + l : 0)];
+          off += 9LL * ld;
         }
         real v = real(0);
 #pragma unroll
@@ -659,7 +669,7 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
 #pragma unroll
         for (int p = 0; p < 9; ++p) v[0][r][p] = real(0);
       gemmRows<1, R, kProd>(v, mats + S::fp + static_cast<int64_t>(h) * B2 * B2, 0, B2, xu, B2, t, TPE, B2);
-      const real* A = aminus + (active ? e : 0) * 324 + 81 * f;
+      const real* A = aminus + (active ? e : 0) * lay.aminusStride + 81 * f;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int row = t + r * TPE;
@@ -689,7 +699,7 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
       const int row = t + r * TPE;
       if (row < B)
 #pragma unroll
-        for (int p = 0; p < 9; ++p) dofs[e * S::QS + p * B + row] += acc[0][r][p];
+        for (int p = 0; p < 9; ++p) dofs[e * lay.qStride + p * lay.ldq + row] += acc[0][r][p];
     }
   }
 }
@@ -697,7 +707,7 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
 // Taylor sum out = sum_i coeffs[i] dQ_i (KernelsAux.cpp:117-184 semantics)
 template <typename real, int O>
 __global__ __launch_bounds__(256) void kTaylor(int64_t n, Coeffs<O, real> cf, const real* __restrict__ deriv,
-                                               int64_t dsize, real* __restrict__ out) {
+                                               CellLayout lay, real* __restrict__ out) {
   constexpr int B = basis3(O);
   const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= n * 9 * B) return;
@@ -708,11 +718,11 @@ __global__ __launch_bounds__(256) void kTaylor(int64_t n, Coeffs<O, real> cf, co
   int64_t off = 0;
 #pragma unroll
   for (int d = 0; d < O; ++d) {
-    const int m = basis3(O - d);
-    if (k < m) acc += cf.c[d] * deriv[e * dsize + off + p * m + k];
-    off += 9 * m;
+    const int m = basis3(O - d), ld = lay.rows(m);
+    if (k < m) acc += cf.c[d] * deriv[e * lay.dsize + off + p * ld + k];
+    off += 9LL * ld;
   }
-  out[idx] = acc;
+  out[e * lay.qStride + p * lay.ldq + k] = acc;
 }
 
 template <typename real, int O>
@@ -720,12 +730,6 @@ Coeffs<O, real> makeCoeffs(const double* c) {
   Coeffs<O, real> r{};
   for (int i = 0; i < O; ++i) r.c[i] = static_cast<real>(c[i]);
   return r;
-}
-
-int64_t derivativesSize(int order) {
-  int64_t s = 0;
-  for (int i = 0; i < order; ++i) s += 9 * basis3(order - i);
-  return s;
 }
 
 template <typename real, int O, int RX = 0, int WV = 4>
@@ -738,15 +742,16 @@ int aderImpl(const EngineState& st, const double* coeffs, const sdg_layer* L, hi
   const auto cf = makeCoeffs<real, O>(coeffs);
   const auto* mats = static_cast<const real*>(st.matrices);
   const LtsArgs lts{L->lts_setup, reset};
+  const CellLayout lay = cellLayout(st, L);
   if (withLocal)
     kAder<real, O, true, RX, WV><<<grid, S::THREADS, 0, stream>>>(
         n, mats, cf, static_cast<real*>(L->dofs), static_cast<real*>(L->time_integrated),
-        static_cast<real*>(L->derivatives), derivativesSize(O), static_cast<const real*>(L->star),
+        static_cast<real*>(L->derivatives), lay, static_cast<const real*>(L->star),
         static_cast<const real*>(L->aplus), L->face_types, lts);
   else
     kAder<real, O, false, RX, WV><<<grid, S::THREADS, 0, stream>>>(
         n, mats, cf, static_cast<real*>(L->dofs), static_cast<real*>(L->time_integrated),
-        static_cast<real*>(L->derivatives), derivativesSize(O), static_cast<const real*>(L->star),
+        static_cast<real*>(L->derivatives), lay, static_cast<const real*>(L->star),
         static_cast<const real*>(L->aplus), L->face_types, lts);
   SDG_HIP_CHECK(hipGetLastError());
   return SDG_OK;
@@ -761,7 +766,7 @@ int localImpl(const EngineState& st, const sdg_layer* L, hipStream_t stream) {
   kLocal<real, O><<<grid, S::THREADS, 0, stream>>>(
       n, static_cast<const real*>(st.matrices), static_cast<real*>(L->dofs),
       static_cast<const real*>(L->time_integrated), static_cast<const real*>(L->star),
-      static_cast<const real*>(L->aplus), L->face_types);
+      static_cast<const real*>(L->aplus), L->face_types, cellLayout(st, L));
   SDG_HIP_CHECK(hipGetLastError());
   return SDG_OK;
 }
@@ -777,25 +782,25 @@ int neighborImpl(const EngineState& st, const sdg_layer* L, hipStream_t stream, 
   if (gtsCoeffs != nullptr) {
     lts.setup = L->lts_setup;
     lts.deriv = static_cast<const real*>(L->derivatives);
-    lts.dsize = derivativesSize(O);
     lts.gts = makeCoeffs<real, O>(gtsCoeffs);
     lts.sub = makeCoeffs<real, O>(subCoeffs);
   }
   kNeighbor<real, O, RX, WV><<<grid, S::THREADS, 0, stream>>>(
       n, static_cast<const real*>(st.matrices), static_cast<real*>(L->dofs),
       static_cast<const real*>(L->time_integrated), L->face_neighbors, L->face_relations,
-      static_cast<const real*>(L->aminus), L->face_types, lts);
+      static_cast<const real*>(L->aminus), L->face_types, lts, cellLayout(st, L));
   SDG_HIP_CHECK(hipGetLastError());
   return SDG_OK;
 }
 
 template <typename real, int O>
-int taylorImpl(const double* coeffs, const void* deriv, void* out, int64_t n, hipStream_t stream) {
+int taylorImpl(const EngineState& st, const double* coeffs, const void* deriv, void* out, int64_t n,
+               hipStream_t stream) {
   if (n == 0) return SDG_OK;
   const int64_t total = n * 9 * basis3(O);
   const dim3 grid(static_cast<unsigned>((total + 255) / 256));
   kTaylor<real, O><<<grid, 256, 0, stream>>>(n, makeCoeffs<real, O>(coeffs), static_cast<const real*>(deriv),
-                                              derivativesSize(O), static_cast<real*>(out));
+                                              engineLayout(st), static_cast<real*>(out));
   SDG_HIP_CHECK(hipGetLastError());
   return SDG_OK;
 }
@@ -877,8 +882,8 @@ struct NeighborOp {
 };
 template <typename real, int O>
 struct TaylorOp {
-  static int run(const EngineState&, const double* c, const void* d, void* o, int64_t n, hipStream_t s) {
-    return taylorImpl<real, O>(c, d, o, n, s);
+  static int run(const EngineState& st, const double* c, const void* d, void* o, int64_t n, hipStream_t s) {
+    return taylorImpl<real, O>(st, c, d, o, n, s);
   }
 };
 

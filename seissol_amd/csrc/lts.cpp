@@ -35,12 +35,6 @@ constexpr uint16_t kBuffers = SDG_LTS_HAS_BUFFERS;
 constexpr uint16_t kDerivatives = SDG_LTS_HAS_DERIVATIVES;
 constexpr uint16_t kAccumulate = SDG_LTS_ACCUMULATE_BUFFERS;
 
-int64_t derivativesReals(int order) {
-  int64_t s = 0;
-  for (int i = 0; i < order; ++i) s += 9LL * (order - i) * (order - i + 1) * (order - i + 2) / 6;
-  return s;
-}
-
 bool ltsLayerOk(const sdg_layer* L, bool neighbors, const char* who) {
   if (L == nullptr || L->num_elements < 0) {
     setError(std::string(who) + ": invalid layer");
@@ -119,7 +113,8 @@ extern "C" int sdg_compute_local_integration(sdg_engine* engine, const double* c
     return SDG_ERR_INVALID;
   }
   if (!vectorArithmetic(engine, "sdg_compute_local_integration") ||
-      !ltsLayerOk(layer, false, "sdg_compute_local_integration"))
+      !ltsLayerOk(layer, false, "sdg_compute_local_integration") ||
+      !layerStridesOk(engine->s, layer, "sdg_compute_local_integration"))
     return SDG_ERR_INVALID;
   return launchLocalIntegration(engine->s, coeffs, layer, reset_buffers != 0, static_cast<hipStream_t>(stream));
 }
@@ -132,7 +127,8 @@ extern "C" int sdg_compute_neighboring_integration(sdg_engine* engine, const dou
     return SDG_ERR_INVALID;
   }
   if (!vectorArithmetic(engine, "sdg_compute_neighboring_integration") ||
-      !ltsLayerOk(layer, true, "sdg_compute_neighboring_integration"))
+      !ltsLayerOk(layer, true, "sdg_compute_neighboring_integration") ||
+      !layerStridesOk(engine->s, layer, "sdg_compute_neighboring_integration"))
     return SDG_ERR_INVALID;
   return launchNeighboringIntegration(engine->s, time_coeffs, subtime_coeffs, layer,
                                       static_cast<hipStream_t>(stream));
@@ -153,9 +149,9 @@ namespace {
 
 // view of cluster c: own arrays offset to the cluster, neighbour sources global
 sdg_layer clusterView(const sdg_lts& L, int c, bool neighborPass) {
-  const int order = L.engine->s.order;
   const int64_t rs = L.engine->s.arithmetic == SDG_FP64 ? 8 : 4;
-  const int64_t q = 9LL * order * (order + 1) * (order + 2) / 6;
+  const CellLayout lay = cellLayout(L.engine->s, &L.layer);
+  const int64_t q = lay.qStride;
   const int64_t off = L.start[c];
   const sdg_layer& g = L.layer;
   sdg_layer v = g;
@@ -164,9 +160,9 @@ sdg_layer clusterView(const sdg_lts& L, int c, bool neighborPass) {
   };
   v.num_elements = L.start[c + 1] - L.start[c];
   v.dofs = shift(g.dofs, q * rs);
-  v.star = shift(g.star, 243 * rs);
-  v.aplus = shift(g.aplus, 324 * rs);
-  v.aminus = shift(g.aminus, 324 * rs);
+  v.star = shift(g.star, lay.starStride * rs);
+  v.aplus = shift(g.aplus, lay.aplusStride * rs);
+  v.aminus = shift(g.aminus, lay.aminusStride * rs);
   v.face_neighbors = reinterpret_cast<const int32_t*>(shift(g.face_neighbors, 16));
   v.face_relations = reinterpret_cast<const uint8_t*>(shift(g.face_relations, 8));
   v.face_types = reinterpret_cast<const uint8_t*>(shift(g.face_types, 4));
@@ -174,7 +170,7 @@ sdg_layer clusterView(const sdg_lts& L, int c, bool neighborPass) {
   v.face_traces = nullptr;
   if (!neighborPass) {
     v.time_integrated = shift(g.time_integrated, q * rs);
-    v.derivatives = shift(g.derivatives, derivativesReals(order) * rs);
+    v.derivatives = shift(g.derivatives, lay.dsize * rs);
   }
   return v;
 }
@@ -223,7 +219,8 @@ extern "C" int sdg_lts_create(sdg_lts** out, sdg_engine* engine, const sdg_layer
     setError("sdg_lts_create: invalid arguments (1..16 clusters, dt0 > 0)");
     return SDG_ERR_INVALID;
   }
-  if (!vectorArithmetic(engine, "sdg_lts_create") || !ltsLayerOk(layer, true, "sdg_lts_create"))
+  if (!vectorArithmetic(engine, "sdg_lts_create") || !ltsLayerOk(layer, true, "sdg_lts_create") ||
+      !layerStridesOk(engine->s, layer, "sdg_lts_create"))
     return SDG_ERR_INVALID;
   if (cluster_start[0] != 0 || cluster_start[num_clusters] != layer->num_elements) {
     setError("sdg_lts_create: cluster_start must run from 0 to num_elements");
@@ -257,19 +254,28 @@ extern "C" int sdg_lts_autotune(sdg_lts* lts, int repeats, void* stream) {
   const int order = lts->engine->s.order;
   const int64_t n = lts->layer.num_elements;
   const int64_t rs = lts->engine->s.arithmetic == SDG_FP64 ? 8 : 4;
-  const int64_t qBytes = n * 9LL * order * (order + 1) * (order + 2) / 6 * rs;
-  const int64_t dBytes = n * derivativesReals(order) * rs;
+  const CellLayout lay = cellLayout(lts->engine->s, &lts->layer);
+  const int64_t qBytes = n * lay.qStride * rs;
+  const int64_t dBytes = n * lay.dsize * rs;
   void* saved[3] = {nullptr, nullptr, nullptr};
   void* live[3] = {lts->layer.dofs, lts->layer.time_integrated, lts->layer.derivatives};
   const int64_t bytes[3] = {qBytes, qBytes, dBytes};
-  for (int i = 0; i < 3; ++i) {
-    SDG_HIP_CHECK(hipMalloc(&saved[i], bytes[i]));
-    SDG_HIP_CHECK(hipMemcpyAsync(saved[i], live[i], bytes[i], hipMemcpyDeviceToDevice, st));
-  }
-  hipEvent_t e0, e1;
-  SDG_HIP_CHECK(hipEventCreate(&e0));
-  SDG_HIP_CHECK(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = SDG_OK;
+  // every failure from here on goes through the one cleanup path below: the
+  // state is restored if it was saved, and what was allocated is released
+  for (int i = 0; i < 3 && rc == SDG_OK; ++i) {
+    hipError_t err = hipMalloc(&saved[i], bytes[i]);
+    if (err != hipSuccess) saved[i] = nullptr;
+    if (err == hipSuccess) err = hipMemcpyAsync(saved[i], live[i], bytes[i], hipMemcpyDeviceToDevice, st);
+    if (err != hipSuccess) rc = hipFail(err, "sdg_lts_autotune: saving the state");
+  }
+  if (rc == SDG_OK) {
+    hipError_t err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    if (err != hipSuccess) rc = hipFail(err, "sdg_lts_autotune: events");
+  }
+  const bool stateSaved = rc == SDG_OK;
   const int C = static_cast<int>(lts->start.size()) - 1;
   for (int c = 0; c < C && rc == SDG_OK; ++c) {
     const double dtc = lts->dt0 * static_cast<double>(int64_t(1) << c);
@@ -301,16 +307,18 @@ extern "C" int sdg_lts_autotune(sdg_lts* lts, int repeats, void* stream) {
       (kind == 0 ? lts->tileLocal : lts->tileNeighbor)[c] = bestV;
     }
   }
-  for (int i = 0; i < 3; ++i) {
-    if (rc == SDG_OK) {
+  // restore whenever the state was saved, also after a failed tuning launch
+  // (the first error is the one reported)
+  if (stateSaved)
+    for (int i = 0; i < 3; ++i) {
       const hipError_t err = hipMemcpyAsync(live[i], saved[i], bytes[i], hipMemcpyDeviceToDevice, st);
-      if (err != hipSuccess) rc = hipFail(err, "sdg_lts_autotune restore");
+      if (err != hipSuccess && rc == SDG_OK) rc = hipFail(err, "sdg_lts_autotune restore");
     }
-  }
   (void)hipStreamSynchronize(st);
-  for (int i = 0; i < 3; ++i) (void)hipFree(saved[i]);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  for (int i = 0; i < 3; ++i)
+    if (saved[i] != nullptr) (void)hipFree(saved[i]);
+  if (e0 != nullptr) (void)hipEventDestroy(e0);
+  if (e1 != nullptr) (void)hipEventDestroy(e1);
   // a captured macro step used the previous tiles
   if (lts->exec != nullptr) {
     (void)hipGraphExecDestroy(lts->exec);

@@ -1,6 +1,6 @@
 // Training kernels of the friction warm-start MLP (6-32-32-1, tanh, FP32):
 // mean-squared-error loss and gradient over a batch, Adam update, forward.
-// A block of 256 threads takes 256 samples: forward and backward per sample
+// A block of kThreads (128) threads takes 128 samples: forward and backward per sample
 // in registers, the per-sample hidden activations and deltas in LDS, then
 // each thread reduces a slice of the weight gradients over the block's
 // samples and adds it to the global gradient with one atomic per entry.

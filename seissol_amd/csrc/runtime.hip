@@ -207,8 +207,49 @@ int uploadMfma(EngineState& s, const double* gm) {
   }
 }
 
-bool validLayer(const sdg_layer* L, bool needNeighbors) {
+}  // namespace
+
+CellLayout engineLayout(const EngineState& e) {
+  CellLayout c;
+  const int O = e.order;
+  const int rs = e.arithmetic == SDG_FP64 ? 8 : 4;
+  c.align = e.alignment > rs ? e.alignment / rs : 1;
+  c.ldq = c.rows(numBasis3(O));
+  c.qStride = 9LL * c.ldq;
+  c.starCompact = e.starFormat == SDG_STAR_COMPACT ? 1 : 0;
+  c.starStride = c.starCompact ? 72 : 243;
+  for (int i = 0; i < O; ++i) c.dsize += 9LL * c.rows(numBasis3(O - i));
+  return c;
+}
+
+CellLayout cellLayout(const EngineState& e, const sdg_layer* L) {
+  CellLayout c = engineLayout(e);
+  if (L != nullptr) {
+    if (L->star_stride > 0) c.starStride = L->star_stride;
+    if (L->aplus_stride > 0) c.aplusStride = L->aplus_stride;
+    if (L->aminus_stride > 0) c.aminusStride = L->aminus_stride;
+  }
+  return c;
+}
+
+bool layerStridesOk(const EngineState& e, const sdg_layer* L, const char* who) {
+  if (L == nullptr) return true;
+  const int64_t starMin = e.starFormat == SDG_STAR_COMPACT ? 72 : 243;
+  if (L->star_stride < 0 || L->aplus_stride < 0 || L->aminus_stride < 0 ||
+      (L->star_stride > 0 && L->star_stride < starMin) || (L->aplus_stride > 0 && L->aplus_stride < 324) ||
+      (L->aminus_stride > 0 && L->aminus_stride < 324)) {
+    setError(std::string(who) + ": per-cell strides must be 0 (dense) or at least the block size (star " +
+             std::to_string(starMin) + ", aplus/aminus 324 reals)");
+    return false;
+  }
+  return true;
+}
+
+namespace {
+
+bool validLayer(const EngineState& st, const sdg_layer* L, bool needNeighbors) {
   if (L == nullptr) { setError("layer is NULL"); return false; }
+  if (!layerStridesOk(st, L, "layer")) return false;
   if (L->num_elements < 0) { setError("negative element count"); return false; }
   if (L->num_elements > (int64_t(1) << 31) - 1) { setError("too many elements for 32-bit neighbour ids"); return false; }
   if (L->num_elements == 0) return true;
@@ -405,6 +446,31 @@ int sdg_engine_set_global_matrices(sdg_engine* engine, const double* host_matric
   return uploadMfma(engine->s, host_matrices);
 }
 
+int sdg_engine_set_layout(sdg_engine* engine, int alignment, int star_format) {
+  if (engine == nullptr) { setError("sdg_engine_set_layout: engine is NULL"); return SDG_ERR_INVALID; }
+  const int rs = engine->s.arithmetic == SDG_FP64 ? 8 : 4;
+  if (alignment < 0 || alignment > 1024 || (alignment > 0 && ((alignment & (alignment - 1)) != 0 || alignment % rs != 0))) {
+    setError("sdg_engine_set_layout: alignment must be 0 or a power of two, a multiple of sizeof(real), <= 1024");
+    return SDG_ERR_INVALID;
+  }
+  if (star_format != SDG_STAR_DENSE && star_format != SDG_STAR_COMPACT) {
+    setError("sdg_engine_set_layout: star_format must be SDG_STAR_DENSE or SDG_STAR_COMPACT");
+    return SDG_ERR_INVALID;
+  }
+  engine->s.alignment = alignment;
+  engine->s.starFormat = star_format;
+  return SDG_OK;
+}
+
+int sdg_layout_sizes(const sdg_engine* engine, int32_t* dofs_ld, int64_t* dofs_reals, int64_t* derivatives_reals) {
+  if (engine == nullptr) { setError("sdg_layout_sizes: engine is NULL"); return SDG_ERR_INVALID; }
+  const CellLayout c = engineLayout(engine->s);
+  if (dofs_ld) *dofs_ld = c.ldq;
+  if (dofs_reals) *dofs_reals = c.qStride;
+  if (derivatives_reals) *derivatives_reals = c.dsize;
+  return SDG_OK;
+}
+
 int sdg_engine_destroy(sdg_engine* engine) {
   if (engine == nullptr) return SDG_OK;
   if (engine->s.matrices) SDG_HIP_CHECK(hipFree(engine->s.matrices));
@@ -417,20 +483,26 @@ int sdg_engine_destroy(sdg_engine* engine) {
 
 int sdg_compute_batched_ader(sdg_engine* engine, const double* coeffs, double /*time_step_width*/,
                              const sdg_layer* layer, void* stream) {
-  if (engine == nullptr || coeffs == nullptr || !validLayer(layer, false)) return SDG_ERR_INVALID;
+  if (engine == nullptr || coeffs == nullptr || !validLayer(engine->s, layer, false)) return SDG_ERR_INVALID;
   if (layer->num_elements > 0 && layer->star == nullptr) { setError("star matrices missing"); return SDG_ERR_INVALID; }
   return launchAder(engine->s, coeffs, layer, static_cast<hipStream_t>(stream));
 }
 
 int sdg_compute_batched_ader_local(sdg_engine* engine, const double* coeffs, double /*time_step_width*/,
                                    const sdg_layer* layer, void* stream) {
-  if (engine == nullptr || coeffs == nullptr || !validLayer(layer, false)) return SDG_ERR_INVALID;
+  if (engine == nullptr || coeffs == nullptr || !validLayer(engine->s, layer, false)) return SDG_ERR_INVALID;
   if (layer->num_elements > 0 && (layer->star == nullptr || layer->aplus == nullptr)) {
     setError("star/aplus matrices missing");
     return SDG_ERR_INVALID;
   }
-  if (matrixCore(engine->s.arithmetic) && layer->face_traces != nullptr)
+  if (matrixCore(engine->s.arithmetic) && layer->face_traces != nullptr) {
+    if (layer->derivatives != nullptr) {
+      setError("sdg_compute_batched_ader_local: the matrix-core local pass writes face traces, not derivatives; "
+               "pass derivatives = NULL (or face_traces = NULL for the VALU pass that writes them)");
+      return SDG_ERR_UNSUPPORTED;
+    }
     return launchLocalMfma(engine->s, coeffs, layer, true, static_cast<hipStream_t>(stream));
+  }
   return launchAderLocal(engine->s, coeffs, layer, static_cast<hipStream_t>(stream));
 }
 
@@ -446,7 +518,7 @@ int sdg_evaluate_batched(sdg_engine* engine, const double* coeffs, const void* d
 }
 
 int sdg_compute_batched_integral(sdg_engine* engine, const sdg_layer* layer, void* stream) {
-  if (engine == nullptr || !validLayer(layer, false)) return SDG_ERR_INVALID;
+  if (engine == nullptr || !validLayer(engine->s, layer, false)) return SDG_ERR_INVALID;
   if (layer->num_elements > 0 && (layer->star == nullptr || layer->aplus == nullptr)) {
     setError("star/aplus matrices missing");
     return SDG_ERR_INVALID;
@@ -455,7 +527,7 @@ int sdg_compute_batched_integral(sdg_engine* engine, const sdg_layer* layer, voi
 }
 
 int sdg_compute_batched_neighbors_integral(sdg_engine* engine, const sdg_layer* layer, void* stream) {
-  if (engine == nullptr || !validLayer(layer, true)) return SDG_ERR_INVALID;
+  if (engine == nullptr || !validLayer(engine->s, layer, true)) return SDG_ERR_INVALID;
   if (matrixCore(engine->s.arithmetic) && layer->face_traces != nullptr)
     return launchNeighborMfma(engine->s, layer, static_cast<hipStream_t>(stream));
   return launchNeighbor(engine->s, layer, static_cast<hipStream_t>(stream));
@@ -474,7 +546,7 @@ int sdg_proxy_run(sdg_engine* engine, int kernel_id, int steps, double dt, const
     setError("sdg_proxy_run: the DR kernels need DR faces; call sdg_proxy_run_dr");
     return SDG_ERR_INVALID;
   }
-  if (engine == nullptr || steps < 0 || kernel_id < 0 || kernel_id > 4 || !validLayer(layer, kernel_id == 0 || kernel_id == 2)) {
+  if (engine == nullptr || steps < 0 || kernel_id < 0 || kernel_id > 4 || !validLayer(engine->s, layer, kernel_id == 0 || kernel_id == 2)) {
     if (engine == nullptr || steps < 0 || kernel_id < 0 || kernel_id > 4) setError("sdg_proxy_run: invalid arguments");
     return SDG_ERR_INVALID;
   }

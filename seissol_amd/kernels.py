@@ -23,6 +23,12 @@ from ._capi import Layer, check, lib, ptr
 ARITHMETIC = {"fp64": _capi.SDG_FP64, "fp32": _capi.SDG_FP32, "bf16x3": _capi.SDG_BF16X3,
               "bf16x6": _capi.SDG_BF16X6, "f16x3": _capi.SDG_F16X3}
 MATRIX_CORE = ("bf16x3", "bf16x6", "f16x3")
+STAR_FORMAT = {"dense": _capi.SDG_STAR_DENSE, "compact": _capi.SDG_STAR_COMPACT}
+# column-major positions q + 9 p of the 24 non-zeros of an elastic star
+# matrix, column by column with rows ascending (codegen/matrices/star.xml):
+# the order of the compact storage
+STAR_NONZEROS = np.array([6, 7, 8, 15, 16, 17, 24, 25, 26, 33, 34, 43, 44, 51, 53, 54, 57, 59, 64, 66, 67, 74, 76,
+                          77])
 
 
 def num_basis(order: int) -> int:
@@ -83,6 +89,29 @@ class Engine:
         handle = C.c_void_p()
         check(lib().sdg_engine_create(C.byref(handle), order, ARITHMETIC[arithmetic]), "sdg_engine_create")
         self.handle = handle
+
+    def set_layout(self, alignment: int = 0, star_format: str = "dense"):
+        """Storage layout of the cells' arrays (``sdg_engine_set_layout``)."""
+        check(lib().sdg_engine_set_layout(self.handle, int(alignment), STAR_FORMAT[star_format]),
+              "sdg_engine_set_layout")
+        self.alignment, self.star_format = int(alignment), star_format
+
+    def layout_sizes(self) -> tuple:
+        """(ld of a DOF-like block, reals per cell of it, reals per cell of the derivatives)."""
+        ld, q, d = C.c_int32(), C.c_int64(), C.c_int64()
+        check(lib().sdg_layout_sizes(self.handle, C.byref(ld), C.byref(q), C.byref(d)), "sdg_layout_sizes")
+        return ld.value, q.value, d.value
+
+    def derivative_blocks(self) -> list:
+        """(offset, ld, rows) of each derivative block under the engine's layout."""
+        out, off = [], 0
+        a = max(1, getattr(self, "alignment", 0) // (8 if self.arithmetic == "fp64" else 4))
+        for i in range(self.order):
+            m = num_basis(self.order - i)
+            ld = (m + a - 1) // a * a
+            out.append((off, ld, m))
+            off += 9 * ld
+        return out
 
     @property
     def real_dtype(self):

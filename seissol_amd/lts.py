@@ -135,9 +135,9 @@ def derive_lts_setups(cluster_ids, neighbors, face_types=None) -> np.ndarray:
 class DeviceLtsLayer(DeviceLayer):
     """Device layer with derivatives and LTS bits."""
 
-    def __init__(self, engine: Engine, lts: LtsSetup, device: str = "cuda"):
+    def __init__(self, engine: Engine, lts: LtsSetup, device: str = "cuda", struct_arrays: bool = False):
         import torch
-        super().__init__(engine, lts.setup, with_derivatives=True, device=device)
+        super().__init__(engine, lts.setup, with_derivatives=True, device=device, struct_arrays=struct_arrays)
         self.lts = lts
         self.lts_setups = torch.from_numpy(lts.lts_setups.view(np.int16)).to(device)
         self.view.lts_setup = ptr(self.lts_setups)

@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _capi
 from ._capi import DrLayer, Layer, ProxyDr, check, lib, ptr
-from .kernels import MATRIX_CORE, Engine, derivatives_size, num_basis, padded_face_basis
+from .kernels import MATRIX_CORE, STAR_NONZEROS, Engine, num_basis, padded_face_basis
 
 KERNELS = ("all", "local", "neigh", "ader", "localwoader", "neigh_dr", "godunov_dr", "all_dr")
 DR_KERNELS = ("neigh_dr", "godunov_dr", "all_dr")
@@ -96,14 +96,34 @@ def make_setup(order: int, n: int | None = None, mesh: str = "proxy", seed: int 
     return Setup(order, n, vertices, material, neighbors, rel, face_types, dofs, dt, seed, mesh, dims)
 
 
+def struct_reals(nreals: int, real_bytes: int, alignment: int) -> int:
+    """Reals per element of an array of structs {real data[nreals]; Empty e;}
+    aligned to `alignment` bytes (the reference's alignas(Alignment)
+    LocalIntegrationData / NeighboringIntegrationData with the empty elastic
+    specific data, src/Initializer/Typedefs.h:186-203)."""
+    a = max(alignment, real_bytes)
+    size = nreals * real_bytes + 1
+    return (size + a - 1) // a * a // real_bytes
+
+
 class DeviceLayer:
     """Device arrays of one layer plus the ``sdg_layer`` view handed to the C ABI.
 
     The per-element operators are computed on the device in FP64
     (``sdg_cell_local_matrices``) and converted to the engine's real type.
+
+    With ``struct_arrays=True`` the layer is stored as the reference stores it
+    under the engine's layout (``Engine.set_layout``): DOF-like blocks
+    [n][9][ld] and aligned derivative blocks, the star matrices and AplusT
+    inside one array of LocalIntegrationData-like structs and AminusT inside
+    one of NeighboringIntegrationData-like structs (passed with their
+    strides).  Every padding entry holds NaN, so a kernel that read one would
+    spoil its result.  ``packed()`` returns the [n][9][B] view of a DOF-like
+    array.
     """
 
-    def __init__(self, engine: Engine, setup: Setup, with_derivatives: bool = False, device: str = "cuda"):
+    def __init__(self, engine: Engine, setup: Setup, with_derivatives: bool = False, device: str = "cuda",
+                 struct_arrays: bool = False):
         import torch
 
         self.engine = engine
@@ -125,24 +145,59 @@ class DeviceLayer:
                                             ptr(self.face_types), ptr(star64), ptr(aplus64), ptr(aminus64),
                                             stream), "sdg_cell_local_matrices")
         self.star64, self.aplus64, self.aminus64 = star64, aplus64, aminus64
-        if rt == torch.float64:
+        ld, qreals, dreals = engine.layout_sizes()
+        self.ld = ld
+        compact = getattr(engine, "star_format", "dense") == "compact"
+        star_src = star64[:, :, torch.from_numpy(STAR_NONZEROS).to(dev)] if compact else star64
+        strides = (0, 0, 0)
+        nan = float("nan")
+        if struct_arrays:
+            rs = 8 if rt == torch.float64 else 4
+            align = getattr(engine, "alignment", 0)
+            sreals = star_src[0].numel()
+            lstride = struct_reals(sreals + 324, rs, align)
+            nstride = struct_reals(324, rs, align)
+            self.local_data = torch.full((n, lstride), nan, dtype=rt, device=dev)
+            self.local_data[:, :sreals] = star_src.reshape(n, -1).to(rt)
+            self.local_data[:, sreals:sreals + 324] = aplus64.reshape(n, -1).to(rt)
+            self.neighbor_data = torch.full((n, nstride), nan, dtype=rt, device=dev)
+            self.neighbor_data[:, :324] = aminus64.reshape(n, -1).to(rt)
+            self.star = self.local_data
+            self.aplus = self.local_data[:, sreals:]
+            self.aminus = self.neighbor_data
+            strides = (lstride, lstride, nstride)
+        elif rt == torch.float64 and not compact:
             self.star, self.aplus, self.aminus = star64, aplus64, aminus64
         else:
-            self.star, self.aplus, self.aminus = star64.to(rt), aplus64.to(rt), aminus64.to(rt)
-        self.dofs = torch.from_numpy(setup.dofs).to(dev).to(rt).contiguous()
-        self.time_integrated = torch.zeros((n, 9, B), dtype=rt, device=dev)
-        self.derivatives = (torch.zeros((n, derivatives_size(setup.order)), dtype=rt, device=dev)
-                            if with_derivatives else None)
+            self.star, self.aplus, self.aminus = (star_src.contiguous().to(rt), aplus64.to(rt), aminus64.to(rt))
+        fill = nan if struct_arrays else 0.0
+        self.dofs = torch.full((n, 9, ld), fill, dtype=rt, device=dev)
+        self.dofs[:, :, :B] = torch.from_numpy(setup.dofs).to(dev).to(rt)
+        self.time_integrated = torch.full((n, 9, ld), fill, dtype=rt, device=dev)
+        self.time_integrated[:, :, :B] = 0.0
+        self.derivatives = (torch.full((n, dreals), fill, dtype=rt, device=dev) if with_derivatives else None)
         B2P = padded_face_basis(setup.order)
         self.face_traces = (torch.zeros((n, 4, 9, B2P), dtype=torch.float32, device=dev)
                             if engine.arithmetic in MATRIX_CORE else None)
         self.view = Layer(n, ptr(self.dofs), ptr(self.time_integrated), ptr(self.derivatives), ptr(self.star),
                           ptr(self.aplus), ptr(self.aminus), ptr(self.neighbors), ptr(self.face_relations),
-                          ptr(self.face_types), ptr(self.face_traces))
+                          ptr(self.face_types), ptr(self.face_traces), None, *strides)
+
+    def packed(self, t):
+        """The [n][9][B] entries of a DOF-like array of this layer."""
+        return t[:, :, :num_basis(self.setup.order)]
+
+    def derivatives_packed(self):
+        """The derivatives as [n][sum_i 9 B(O-i)] (blocks [9][B(O-i)]), the oracle's layout."""
+        import torch
+        n = self.setup.n
+        parts = [self.derivatives[:, off:off + 9 * ld].reshape(n, 9, ld)[:, :, :m].reshape(n, 9 * m)
+                 for off, ld, m in self.engine.derivative_blocks()]
+        return torch.cat(parts, 1)
 
     def reset_dofs(self, dofs: np.ndarray):
         import torch
-        self.dofs.copy_(torch.from_numpy(dofs).to(self.dofs.device).to(self.dofs.dtype))
+        self.packed(self.dofs).copy_(torch.from_numpy(dofs).to(self.dofs.device).to(self.dofs.dtype))
 
 
 class DeviceDrProxy:
