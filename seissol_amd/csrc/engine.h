@@ -72,6 +72,8 @@ struct EngineState {
 
 // the layout of `layer` (its strides, 0 = the dense defaults) under the engine's settings
 CellLayout cellLayout(const EngineState& e, const sdg_layer* layer);
+// proxy kernels 0-4 on a validated layer (sdg_proxy_run); allocates nothing
+int proxyRun(const EngineState& st, int kernelId, int steps, double dt, const sdg_layer* layer, hipStream_t s);
 // the same for arrays handed over without a layer (derivatives, Taylor outputs)
 CellLayout engineLayout(const EngineState& e);
 // checks the per-cell strides of `layer` against the minimum sizes; sets the error text

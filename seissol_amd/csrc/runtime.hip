@@ -253,7 +253,13 @@ bool validLayer(const EngineState& st, const sdg_layer* L, bool needNeighbors) {
   if (L->num_elements < 0) { setError("negative element count"); return false; }
   if (L->num_elements > (int64_t(1) << 31) - 1) { setError("too many elements for 32-bit neighbour ids"); return false; }
   if (L->num_elements == 0) return true;
-  if (L->dofs == nullptr || L->time_integrated == nullptr) { setError("dofs/time_integrated missing"); return false; }
+  // the matrix-core forms pass face traces between the passes: there the
+  // time integral is an optional output of the local pass
+  const bool tracesOnly = matrixCore(st.arithmetic) && L->face_traces != nullptr;
+  if (L->dofs == nullptr || (L->time_integrated == nullptr && !tracesOnly)) {
+    setError("dofs/time_integrated missing");
+    return false;
+  }
   if (needNeighbors && (L->face_neighbors == nullptr || L->face_relations == nullptr || L->aminus == nullptr)) {
     setError("neighbour data missing");
     return false;
@@ -288,6 +294,43 @@ __global__ void kConvert(int64_t n, const double* __restrict__ src, Dst* __restr
 }
 
 }  // namespace
+
+// the kernels of proxy kernels 0-4 (validated layer); no allocation
+int proxyRun(const EngineState& st, int kernel_id, int steps, double dt, const sdg_layer* layer, hipStream_t s) {
+  double coeffs[kMaxOrder];
+  sdg_monomial_integrate(st.order, 0.0, dt, coeffs);
+  const bool mc = matrixCore(st.arithmetic);
+  if (mc && layer->num_elements > 0 && layer->face_traces == nullptr && (kernel_id == 0 || kernel_id == 1 || kernel_id == 2)) {
+    setError("sdg_proxy_run: matrix-core arithmetic needs layer->face_traces");
+    return SDG_ERR_INVALID;
+  }
+  if (layer->num_elements > 0 && layer->time_integrated == nullptr && (!mc || kernel_id >= 3)) {
+    setError("sdg_proxy_run: this kernel needs layer->time_integrated");
+    return SDG_ERR_INVALID;
+  }
+  for (int step = 0; step < steps; ++step) {
+    int rc = SDG_OK;
+    if (mc && kernel_id <= 2) {
+      if (kernel_id == 0 || kernel_id == 1) rc = launchLocalMfma(st, coeffs, layer, false, s);
+      if (rc == SDG_OK && (kernel_id == 0 || kernel_id == 2)) rc = launchNeighborMfma(st, layer, s);
+      if (rc != SDG_OK) return rc;
+      continue;
+    }
+    switch (kernel_id) {
+      case 0:
+        rc = launchAderLocal(st, coeffs, layer, s);
+        if (rc == SDG_OK) rc = launchNeighbor(st, layer, s);
+        break;
+      case 1: rc = launchAderLocal(st, coeffs, layer, s); break;
+      case 2: rc = launchNeighbor(st, layer, s); break;
+      case 3: rc = launchAder(st, coeffs, layer, s); break;
+      case 4: rc = launchLocal(st, layer, s); break;
+    }
+    if (rc != SDG_OK) return rc;
+  }
+  return SDG_OK;
+}
+
 }  // namespace sdg
 
 using namespace sdg;
@@ -484,7 +527,10 @@ int sdg_engine_destroy(sdg_engine* engine) {
 int sdg_compute_batched_ader(sdg_engine* engine, const double* coeffs, double /*time_step_width*/,
                              const sdg_layer* layer, void* stream) {
   if (engine == nullptr || coeffs == nullptr || !validLayer(engine->s, layer, false)) return SDG_ERR_INVALID;
-  if (layer->num_elements > 0 && layer->star == nullptr) { setError("star matrices missing"); return SDG_ERR_INVALID; }
+  if (layer->num_elements > 0 && (layer->star == nullptr || layer->time_integrated == nullptr)) {
+    setError("star matrices or time_integrated missing");
+    return SDG_ERR_INVALID;
+  }
   return launchAder(engine->s, coeffs, layer, static_cast<hipStream_t>(stream));
 }
 
@@ -519,8 +565,9 @@ int sdg_evaluate_batched(sdg_engine* engine, const double* coeffs, const void* d
 
 int sdg_compute_batched_integral(sdg_engine* engine, const sdg_layer* layer, void* stream) {
   if (engine == nullptr || !validLayer(engine->s, layer, false)) return SDG_ERR_INVALID;
-  if (layer->num_elements > 0 && (layer->star == nullptr || layer->aplus == nullptr)) {
-    setError("star/aplus matrices missing");
+  if (layer->num_elements > 0 &&
+      (layer->star == nullptr || layer->aplus == nullptr || layer->time_integrated == nullptr)) {
+    setError("star/aplus matrices or time_integrated missing");
     return SDG_ERR_INVALID;
   }
   return launchLocal(engine->s, layer, static_cast<hipStream_t>(stream));
@@ -550,35 +597,7 @@ int sdg_proxy_run(sdg_engine* engine, int kernel_id, int steps, double dt, const
     if (engine == nullptr || steps < 0 || kernel_id < 0 || kernel_id > 4) setError("sdg_proxy_run: invalid arguments");
     return SDG_ERR_INVALID;
   }
-  std::vector<double> coeffs(engine->s.order);
-  sdg_monomial_integrate(engine->s.order, 0.0, dt, coeffs.data());
-  auto st = static_cast<hipStream_t>(stream);
-  const bool mc = matrixCore(engine->s.arithmetic);
-  if (mc && layer->num_elements > 0 && layer->face_traces == nullptr && (kernel_id == 0 || kernel_id == 1 || kernel_id == 2)) {
-    setError("sdg_proxy_run: matrix-core arithmetic needs layer->face_traces");
-    return SDG_ERR_INVALID;
-  }
-  for (int s = 0; s < steps; ++s) {
-    int rc = SDG_OK;
-    if (mc && kernel_id <= 2) {
-      if (kernel_id == 0 || kernel_id == 1) rc = launchLocalMfma(engine->s, coeffs.data(), layer, false, st);
-      if (rc == SDG_OK && (kernel_id == 0 || kernel_id == 2)) rc = launchNeighborMfma(engine->s, layer, st);
-      if (rc != SDG_OK) return rc;
-      continue;
-    }
-    switch (kernel_id) {
-      case 0:
-        rc = launchAderLocal(engine->s, coeffs.data(), layer, st);
-        if (rc == SDG_OK) rc = launchNeighbor(engine->s, layer, st);
-        break;
-      case 1: rc = launchAderLocal(engine->s, coeffs.data(), layer, st); break;
-      case 2: rc = launchNeighbor(engine->s, layer, st); break;
-      case 3: rc = launchAder(engine->s, coeffs.data(), layer, st); break;
-      case 4: rc = launchLocal(engine->s, layer, st); break;
-    }
-    if (rc != SDG_OK) return rc;
-  }
-  return SDG_OK;
+  return proxyRun(engine->s, kernel_id, steps, dt, layer, static_cast<hipStream_t>(stream));
 }
 
 int sdg_upload_reals(sdg_engine* engine, void* device_dst, const double* host_src, int64_t count, void* stream) {

@@ -3,7 +3,8 @@
 // Usage (as app/Proxy/Main.cpp:49-66 plus this engine's options):
 //   seissol_amd_proxy [--order O] [--arithmetic fp64|fp32|bf16x3|bf16x6|f16x3]
 //                     [--mesh proxy|periodic] [--seed S] [-f plain|json]
-//                     <cells> <timesteps> <kernel[,kernel...]>
+//                     [--dump FILE] <cells> <timesteps> <kernel[,kernel...]>
+// --dump writes the final DOFs ([cells][9][B], FP64, native byte order).
 // Kernels: all, local, neigh, ader, localwoader, neigh_dr, godunov_dr, all_dr
 // (src/Proxy/Common.cpp:22-41); a comma-separated list runs its kernels in
 // sequence in every pass.  With a DR kernel every cell face is a DR face and
@@ -15,7 +16,13 @@
 // rho ~ U[2000, 3000], vp 6000, vs 3464, DOFs ~ U[-1, 1], dt from CFL 0.5.
 // Timing as src/Proxy/Runner.cpp:57-154: one untimed pass, then `timesteps`
 // passes between device synchronisations; the summary follows
-// Aux::writeOutput (src/Proxy/Common.cpp:87-160).
+// Aux::writeOutput (src/Proxy/Common.cpp:87-160).  The kernels 0-4 run
+// through the C++ kernel classes as the reference's device proxy runs them
+// (src/Proxy/KernelDevice.cpp:32-124): ader = Spacetime::computeBatchedAder,
+// localwoader = Local::computeBatchedIntegral, local = ader fused with the
+// local integral, neigh = Neighbor::computeBatchedNeighborsIntegral, all =
+// local then neigh.  The matrix-core forms' local pass writes the face traces
+// the neighbour pass reads, so there the layer carries no time_integrated.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,6 +30,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
@@ -88,7 +96,8 @@ std::vector<std::string> split(const std::string& s) {
 int usage(const char* msg) {
   if (msg != nullptr) std::cerr << msg << "\n";
   std::cerr << "usage: seissol_amd_proxy [--order O] [--arithmetic fp64|fp32|bf16x3|bf16x6|f16x3] "
-               "[--mesh proxy|periodic] [--seed S] [-f plain|json] <cells> <timesteps> <kernel[,kernel...]>\n"
+               "[--mesh proxy|periodic] [--seed S] [-f plain|json] [--dump FILE] <cells> <timesteps> "
+               "<kernel[,kernel...]>\n"
                "kernels: all, local, neigh, ader, localwoader, neigh_dr, godunov_dr, all_dr\n";
   return -1;
 }
@@ -136,7 +145,7 @@ void writeOutput(std::ostream& s, const Output& o, const std::string& kernelStr,
 
 int run(int argc, char** argv) {
   int order = 6;
-  std::string arith = "fp64", mesh = "proxy", format = "plain";
+  std::string arith = "fp64", mesh = "proxy", format = "plain", dumpPath;
   uint64_t seed = 1234;
   std::vector<std::string> positional;
   for (int i = 1; i < argc; ++i) {
@@ -150,6 +159,7 @@ int run(int argc, char** argv) {
     else if (a == "--mesh") mesh = next("--mesh");
     else if (a == "--seed") seed = std::stoull(next("--seed"));
     else if (a == "-f" || a == "--format") format = next("--format");
+    else if (a == "--dump") dumpPath = next("--dump");
     else if (a == "-h" || a == "--help") return usage(nullptr);
     else positional.push_back(a);
   }
@@ -268,17 +278,39 @@ int run(int argc, char** argv) {
   }
   hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
   sdg_layer layer{cells, dDofs, dInt, dDeriv, star, aplus, aminus, dNb, dRel, dTypes, dTraces, nullptr};
+  // the matrix-core local pass hands face traces to the neighbour pass, not I
+  sdg_layer classLayer = layer;
+  if (traces) classLayer.time_integrated = nullptr;
   std::cerr << "...done" << std::endl;
 
   seissol_amd::StreamRuntime runtime{};
   hipStream_t stream;
   hipCheck(hipStreamCreate(&stream), "hipStreamCreate");
   runtime.stream = stream;
+  seissol_amd::kernels::Spacetime spacetime;
+  seissol_amd::kernels::Local local;
+  seissol_amd::kernels::Neighbor neighbor;
+  spacetime.setGlobalData(global);
+  local.setGlobalData(global);
+  neighbor.setGlobalData(global);
+  std::vector<double> coeffs(order);
+  check(sdg_monomial_integrate(order, 0.0, dt, coeffs.data()), "sdg_monomial_integrate");
   auto run1 = [&](int id, int steps) {
-    if (withDR)
+    if (withDR) {
       check(sdg_proxy_run_dr(global.engine(), id, steps, dt, &layer, &dr, runtime.stream), "sdg_proxy_run_dr");
-    else
-      check(sdg_proxy_run(global.engine(), id, steps, dt, &layer, runtime.stream), "sdg_proxy_run");
+      return;
+    }
+    for (int s = 0; s < steps; ++s) switch (id) {
+        case 0:
+          spacetime.computeBatchedAderLocal(coeffs.data(), dt, classLayer, runtime);
+          neighbor.computeBatchedNeighborsIntegral(classLayer, runtime);
+          break;
+        case 1: spacetime.computeBatchedAderLocal(coeffs.data(), dt, classLayer, runtime); break;
+        case 2: neighbor.computeBatchedNeighborsIntegral(classLayer, runtime); break;
+        case 3: spacetime.computeBatchedAder(coeffs.data(), dt, layer, runtime); break;
+        case 4: local.computeBatchedIntegral(layer, runtime); break;
+        default: throw std::runtime_error("unknown proxy kernel");
+      }
   };
   auto pass = [&](int64_t steps) {
     if (kernelIds.size() == 1) {
@@ -311,6 +343,14 @@ int run(int argc, char** argv) {
             << work / sec << " element-updates/s" << std::endl;
   writeOutput(std::cout, o, kernelStr, format == "json");
   std::cout.flush();
+  if (!dumpPath.empty()) {
+    std::vector<double> out(dofs.size());
+    check(sdg_download_reals(global.engine(), out.data(), dDofs, static_cast<int64_t>(out.size()), nullptr),
+          "sdg_download_reals");
+    std::ofstream f(dumpPath, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(out.data()), static_cast<std::streamsize>(out.size() * sizeof(double)));
+    if (!f) throw std::runtime_error("cannot write " + dumpPath);
+  }
 
   hipStreamDestroy(stream);
   for (void* p : {static_cast<void*>(dVert), static_cast<void*>(dMat), static_cast<void*>(dNb),
