@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -90,23 +91,44 @@ def measured_traffic(form: str, order: int, elements: int, kernel: str):
     return None
 
 
+def fault_cfl_dt(length: float, faces: int, order: int) -> float:
+    """CFL step (GlobalTimestep.cpp:27-47, cfl 0.5, vp 6000) of the corner
+    tetrahedron (0,0,0), (h,0,0), (0,h,0), (0,0,h) on a fault triangle of the
+    square grid of `faces` triangles over a `length` x `length` plane."""
+    from seissol_amd._capi import lib
+    import ctypes as C
+    h = length / max(1, int(round(math.sqrt(faces / 2))))
+    v = np.array([[0, 0, 0], [h, 0, 0], [0, h, 0], [0, 0, h]], dtype=np.float64)
+    m = np.array([2670.0, 2670.0 * 3464.0 ** 2, 2670.0 * (6000.0 ** 2 - 2 * 3464.0 ** 2)], dtype=np.float64)
+    L = lib()
+    L.sdg_cfl_timestep.restype = C.c_double
+    return float(L.sdg_cfl_timestep(1, order, 0.5, v.ctypes.data_as(C.c_void_p), m.ctypes.data_as(C.c_void_p)))
+
+
 def friction_bench(args) -> dict:
     """Rate-and-state friction (law 103) at ORDER 4: plain Newton against the
-    MLP warm start, counted in residual evaluations, on the same fault."""
+    warm start (closed-form creep root, then the MLP), counted in Newton steps
+    and residual evaluations, on the same fault."""
     import torch
 
     from seissol_amd._capi import FrictionOptions, ptr
     from seissol_amd.friction import FaultDriver, FrictionSolver, WarmStartTrainer, make_fault_setup
 
-    steps, dt = args.friction_steps, 1e-3
-    setup = make_fault_setup(num_faces=args.friction_faces, order=4, seed=args.seed)
-    solves = setup.num_points * args.friction_faces * 4 * steps
+    length = 10_000.0
+    dt = fault_cfl_dt(length, args.friction_faces, 4)
+    steps = args.friction_steps
+    setup = make_fault_setup(num_faces=args.friction_faces, order=4, seed=args.seed, length=length)
+    np_ = setup.num_points
+    solves = np_ * args.friction_faces * 4 * steps
     stride = max(1, solves // args.friction_samples)
     cap = args.friction_samples
     buf = torch.zeros((cap, 7), dtype=torch.float32, device="cuda")
     cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
 
-    def run(solver, options=None):
+    def run(closed_form=False, mlp=None, options=None):
+        solver = FrictionSolver(setup.params, 4)
+        solver.closed_form_start = closed_form
+        solver.mlp_params = mlp
         drv = FaultDriver(setup, solver, dt)
         torch.cuda.synchronize()
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -114,51 +136,56 @@ def friction_bench(args) -> dict:
         drv.advance(steps, options)
         e.record()
         torch.cuda.synchronize()
-        return drv, s.elapsed_time(e) / steps
+        ev = int(drv.fault.t["newton_iterations"][:, :np_].sum().item())
+        st = int(drv.fault.t["slip_rate_updates"][:, :np_].sum().item())
+        return drv, {"newton_steps": st, "residual_evaluations": ev, "ms_per_dr_step": s.elapsed_time(e) / steps}
 
     # plain solver: one run recording training samples, one timed run without
-    run(FrictionSolver(setup.params, 4), FrictionOptions(None, ptr(buf), cap, ptr(cnt), stride, 0))
+    run(options=FrictionOptions(None, ptr(buf), cap, ptr(cnt), stride, 0))
     n = min(int(cnt.item()), cap)
-    drv0, ms_plain = run(FrictionSolver(setup.params, 4))
-    plain = int(drv0.fault.t["newton_iterations"].sum().item())
-    log(f"friction plain: {plain} evaluations, {ms_plain:.3f} ms/step, {n} samples")
+    drv0, plain = run()
+    log(f"friction plain: {plain}, {n} samples")
     g = torch.Generator(device="cuda").manual_seed(args.seed)
     perm = torch.randperm(n, generator=g, device="cuda")
     train, held = buf[:n][perm[: n * 19 // 20]].contiguous(), buf[:n][perm[n * 19 // 20:]].contiguous()
     tr = WarmStartTrainer(seed=args.seed)
     t0 = time.perf_counter()
     tr.fit(train, steps=args.friction_train_steps, batch=1024, lr=3e-3, seed=args.seed)
+    torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     held_err = (tr.predict(held) - held[:, 6]).abs()
     off = held[:, 6].abs() > 0.01
-    solver = FrictionSolver(setup.params, 4)
-    solver.mlp_params = tr.params
-    drv1, ms_warm = run(solver)
-    warm = int(drv1.fault.t["newton_iterations"].sum().item())
-    log(f"friction warm: {warm} evaluations, {ms_warm:.3f} ms/step")
+    drv1, closed = run(closed_form=True)
+    log(f"friction closed-form start: {closed}")
+    drv2, warm = run(closed_form=True, mlp=tr.params)
+    log(f"friction closed form + MLP: {warm}")
+    drv3, mlp_only = run(mlp=tr.params)
+    log(f"friction MLP only: {mlp_only}")
     diffs = {}
     for name in ("slip_rate_magnitude", "state_variable", "slip1", "traction1"):
-        a, b = drv1.fault.t[name], drv0.fault.t[name]
+        a, b = drv2.fault.t[name], drv0.fault.t[name]
         diffs[name] = float((a - b).norm() / b.norm().clamp_min(1e-300))
     return {
-        "metric": "Newton residual evaluations, FVW rate-and-state ORDER 4, plain vs MLP warm start",
-        "value": plain / warm, "unit": "x fewer evaluations", "higher_is_better": True,
-        "plain_evaluations": plain, "warm_evaluations": warm,
-        "ms_per_dr_step": {"plain": ms_plain, "warm": ms_warm},
-        "config": {"workload": f"planar fault, {args.friction_faces} faces x {setup.num_points} Gauss points, "
-                               f"ORDER 4, {steps} DR steps of {dt} s, nucleation + travelling shear pulse",
-                   "faces": args.friction_faces, "points": setup.num_points, "steps": steps, "seed": args.seed},
+        "metric": "Newton steps of the slip-rate solves, FVW rate-and-state ORDER 4, plain vs warm start",
+        "value": plain["newton_steps"] / warm["newton_steps"], "unit": "x fewer Newton steps",
+        "higher_is_better": True,
+        "evaluation_cut": plain["residual_evaluations"] / warm["residual_evaluations"],
+        "plain": plain, "closed_form_start": closed, "closed_form_and_mlp": warm, "mlp_only": mlp_only,
+        "config": {"workload": f"planar fault {length / 1000:.0f} x {length / 1000:.0f} km, {args.friction_faces} "
+                               f"faces x {np_} Gauss points, ORDER 4, {steps} DR steps of the CFL step "
+                               f"{dt:.4e} s, nucleation patch + travelling shear pulse",
+                   "faces": args.friction_faces, "points": np_, "steps": steps, "dt": dt, "seed": args.seed},
         "training": {"samples": n, "stride": stride, "adam_steps": args.friction_train_steps, "seconds": train_s,
                      "held_out_median_abs_log10_err": float(held_err.median().item()),
                      "held_out_p90_abs_log10_err": float(held_err.quantile(0.9).item()),
-                     "creep_estimate_median_abs_log10_err": float(held[:, 6].abs().median().item()),
-                     "where_creep_estimate_off": {
+                     "bound_median_abs_log10_err": float(held[:, 6].abs().median().item()),
+                     "where_bound_off": {
                          "fraction": float(off.float().mean().item()),
                          "network_median_abs_log10_err": float(held_err[off].median().item()) if off.any() else None,
-                         "creep_estimate_median_abs_log10_err": float(held[:, 6].abs()[off].median().item())
+                         "bound_median_abs_log10_err": float(held[:, 6].abs()[off].median().item())
                          if off.any() else None}},
         "rel_l2_warm_vs_plain": diffs,
-        "converged": bool(drv1.fault.t["convergence_inner"].all().item()),
+        "converged": bool(drv2.fault.t["convergence_inner"].all().item()),
         "dtype": "f64 (Newton and residual), f32 (MLP)", "data": "synthetic",
     }
 
@@ -246,7 +273,7 @@ def main(argv=None) -> int:
     ap.add_argument("--friction", action="store_true",
                     help="only run the friction benchmark (plain Newton vs MLP warm start)")
     ap.add_argument("--friction-faces", type=int, default=20_000)
-    ap.add_argument("--friction-steps", type=int, default=60)
+    ap.add_argument("--friction-steps", type=int, default=120)
     ap.add_argument("--friction-samples", type=int, default=1_000_000)
     ap.add_argument("--friction-train-steps", type=int, default=4000)
     ap.add_argument("--lts", action="store_true", help="only run the local-time-stepping benchmark")

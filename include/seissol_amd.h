@@ -324,20 +324,31 @@ typedef struct {
   uint8_t* convergence_outer;
   double* imposed_state_plus;        /* [f][9][points]                       */
   double* imposed_state_minus;
-  int32_t* newton_iterations;        /* optional [f][points]: Newton function evaluations, accumulated */
+  int32_t* newton_iterations;        /* optional [f][points]: residual evaluations g(V) of the slip-rate solves, accumulated */
+  int32_t* slip_rate_updates;        /* optional [f][points]: Newton steps (the reference's "slip-rate updates",
+                                        rsMaxNumberSlipRateUpdates bounds them per solve), accumulated */
 } sdg_fault_layer;
 
 /* Options of sdg_friction_evaluate: the warm start of the first slip-rate
- * solve of each time point by a 6-32-32-1 MLP (FP32 parameters, layout of
- * sdg_mlp_param_count()), and the recording of (features, target) training
- * samples from the solves of a run without the MLP. */
+ * solve of each time point, and the recording of (features, target) training
+ * samples from the solves of a run without the MLP.
+ *   closed_form_start != 0: a point whose closed-form creep root
+ *     V_c = 2 V0 exp(-psi/a) sinh(S/N)  (S = invEtaS tau, N = invEtaS |sigma_n| a;
+ *     the root of g(V) + V = 0, so g(V_c) = -V_c) lies below half the slip-rate
+ *     tolerance starts its Newton solve there: its first residual test passes.
+ *   mlp_params != NULL: every other point takes the 6-32-32-1 MLP's predicted
+ *     root (FP32 parameters, layout of sdg_mlp_param_count()) when it lies
+ *     more than a decade from the previous slip rate.
+ * Either way the residual test and the tolerances are unchanged; g is
+ * strictly decreasing, so a converged solve returns the plain root up to the
+ * tolerance. */
 typedef struct {
   const float* mlp_params;           /* device, or NULL                      */
   float* samples;                    /* device [capacity][7] or NULL         */
   int64_t capacity;
   unsigned long long* sample_count;  /* device counter (atomic)              */
   int32_t sample_stride;             /* keep 1 of `stride` steady solves     */
-  int32_t pad_;
+  int32_t closed_form_start;         /* 0: plain start at the previous slip rate */
 } sdg_friction_options;
 
 /* dr::friction_law::FrictionSolver::evaluate for the FVW law on the GPU

@@ -167,7 +167,8 @@ FAULT_FIELDS = ["imp_and_eta", "q_interpolated_plus", "q_interpolated_minus", "i
                 "state_variable", "slip_rate1", "slip_rate2", "slip_rate_magnitude", "mu", "traction1",
                 "traction2", "slip1", "slip2", "accumulated_slip", "rupture_time", "peak_slip_rate",
                 "dyn_stress_time", "rupture_time_pending", "dyn_stress_time_pending", "convergence_inner",
-                "convergence_outer", "imposed_state_plus", "imposed_state_minus", "newton_iterations"]
+                "convergence_outer", "imposed_state_plus", "imposed_state_minus", "newton_iterations",
+                "slip_rate_updates"]
 
 
 class OracleFaultLayer(C.Structure):
@@ -179,7 +180,7 @@ class OracleFrictionParams(C.Structure):
     _fields_ = [("rs_sr0", C.c_double), ("rs_slip_rate_tolerance", C.c_double), ("rs_state_tolerance", C.c_double),
                 ("rs_max_slip_rate_updates", C.c_int32), ("rs_state_variable_updates", C.c_int32),
                 ("eta_damp", C.c_double), ("eta_damp_end", C.c_double), ("nucleation_count", C.c_int32),
-                ("pad_", C.c_int32), ("t0", C.c_double), ("s0", C.c_double)]
+                ("closed_form_start", C.c_int32), ("t0", C.c_double), ("s0", C.c_double)]
 
 
 def _flib():
@@ -213,12 +214,16 @@ class FaultState:
                                      *[(self.a[n].ctypes.data if n in self.a else None) for n in FAULT_FIELDS])
 
 
-def friction_evaluate(state: FaultState, params: dict, full_update_time, delta_t, time_weights, resample):
-    """One friction step of every face (FrictionSolver::evaluate, law 103)."""
+def friction_evaluate(state: FaultState, params: dict, full_update_time, delta_t, time_weights, resample,
+                      closed_form_start: bool = False):
+    """One friction step of every face (FrictionSolver::evaluate, law 103).
+    closed_form_start: creeping points start their first slip-rate solve of a
+    time point at the closed-form creep root (the library's option of the
+    same name); the root is unchanged up to the tolerance."""
     prm = OracleFrictionParams(params["rs_sr0"], params["rs_slip_rate_tolerance"], params["rs_state_tolerance"],
                                params["rs_max_slip_rate_updates"], params["rs_state_variable_updates"],
-                               params["eta_damp"], params["eta_damp_end"], params["nucleation_count"], 0,
-                               params["t0"], params["s0"])
+                               params["eta_damp"], params["eta_damp_end"], params["nucleation_count"],
+                               1 if closed_form_start else 0, params["t0"], params["s0"])
     dt = _c(delta_t)
     tw = _c(time_weights)
     rs = _c(resample)

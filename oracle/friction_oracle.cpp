@@ -11,7 +11,10 @@
 //   initial state       src/DynamicRupture/Initializer/RateAndStateInitializer.cpp:154-180
 // Convergence checks are face-wide over the real Gauss points, as in the CPU
 // implementation (RateAndState.h:209-222, 375-386).  No thermal
-// pressurization (NoTP), elastic material.
+// pressurization (NoTP), elastic material.  Counts residual evaluations and
+// Newton steps per point.  Option closed_form_start: a point whose closed-form
+// creep root lies below half the slip-rate tolerance starts the first solve of
+// a time point there (the library's warm-start option without the network).
 //
 // Also: a FP64 forward/backward of the 6-32-32-1 warm-start MLP used to check
 // the GPU training kernels' gradients.
@@ -61,7 +64,8 @@ typedef struct {
   uint8_t* convergence_outer;
   double* imposed_state_plus;
   double* imposed_state_minus;
-  int32_t* newton_iterations;
+  int32_t* newton_iterations;  // residual evaluations
+  int32_t* slip_rate_updates;  // Newton steps
 } OracleFaultLayer;
 
 typedef struct {
@@ -73,7 +77,7 @@ typedef struct {
   double eta_damp;
   double eta_damp_end;
   int32_t nucleation_count;
-  int32_t pad_;
+  int32_t closed_form_start;  // creeping points start at the closed-form creep root
   double t0;
   double s0;
 } OracleFrictionParams;
@@ -220,6 +224,17 @@ void evaluateFace(const OracleFaultLayer* L, const OracleFrictionParams* prm, in
         normal[p] = std::min(0.0, fsN[o * NP + p] + F.is(0, p) + L->initial_pressure[F.at(p)]);
       // invertSlipRateIterative (RateAndState.h:339-416): face-wide convergence
       for (int p = 0; p < NP; ++p) test[p] = L->slip_rate_magnitude[F.at(p)];
+      if (prm->closed_form_start && j == 0)
+        for (int p = 0; p < NP; ++p) {
+          // V_c = 2 V0 exp(-psi/a) sinh(S/N) solves N asinh(V exp(psi/a) / (2 V0)) = S,
+          // i.e. g(V_c) = -V_c: below half the tolerance the first test passes there
+          const double S = invEtaS * absShear[p], N = invEtaS * std::fabs(normal[p]) * F.a(p);
+          if (S > 0.0 && N > 0.0) {
+            const double c = S / N;
+            const double vc = std::exp(std::log(2.0 * prm->rs_sr0) - sv[p] / F.a(p) + c + std::log(-0.5 * std::expm1(-2.0 * c)));
+            if (vc < 0.5 * prm->rs_slip_rate_tolerance) test[p] = std::max(kAlmostZero, vc);
+          }
+        }
       bool converged = false;
       auto evalAll = [&]() {
         for (int p = 0; p < NP; ++p) {
@@ -235,6 +250,8 @@ void evaluateFace(const OracleFaultLayer* L, const OracleFrictionParams* prm, in
       for (int it = 0; it < prm->rs_max_slip_rate_updates; ++it) {
         converged = evalAll();
         if (converged) break;
+        if (L->slip_rate_updates)
+          for (int p = 0; p < NP; ++p) L->slip_rate_updates[F.at(p)] += 1;
         for (int p = 0; p < NP; ++p) {
           const double dG = -invEtaS * (std::fabs(normal[p]) * F.dmu(p, test[p], sv[p])) - 1.0;
           test[p] = std::max(kAlmostZero, test[p] - g[p] / dG);

@@ -69,7 +69,8 @@ FAULT_FIELDS = ["imp_and_eta", "q_interpolated_plus", "q_interpolated_minus", "i
                 "state_variable", "slip_rate1", "slip_rate2", "slip_rate_magnitude", "mu", "traction1",
                 "traction2", "slip1", "slip2", "accumulated_slip", "rupture_time", "peak_slip_rate",
                 "dyn_stress_time", "rupture_time_pending", "dyn_stress_time_pending", "convergence_inner",
-                "convergence_outer", "imposed_state_plus", "imposed_state_minus", "newton_iterations"]
+                "convergence_outer", "imposed_state_plus", "imposed_state_minus", "newton_iterations",
+                "slip_rate_updates"]
 
 
 class FaultLayer(C.Structure):
@@ -105,7 +106,7 @@ class FrictionOptions(C.Structure):
     """``sdg_friction_options``"""
 
     _fields_ = [("mlp_params", P), ("samples", P), ("capacity", I64), ("sample_count", P),
-                ("sample_stride", C.c_int32), ("pad_", C.c_int32)]
+                ("sample_stride", C.c_int32), ("closed_form_start", C.c_int32)]
 
 
 # name -> (restype, argtypes)

@@ -9,13 +9,14 @@
 // state resampling and the imposed states (FrictionSolverCommon.h:146-631,
 // FastVelocityWeakeningLaw.h:43-165, RateAndStateCommon.h:22-139).
 //
-// Optionally a 6-32-32-1 MLP (FP32) warm-starts the first slip-rate solve of
-// each time point: when the previous slip rate fails the residual test, a
-// point whose predicted root lies more than a decade away (and inside the
-// bracket that the sign of the residual gives) takes the prediction as its
-// next iterate instead of the Newton step.  No evaluation is added and the
-// residual test is unchanged; g is strictly decreasing in V, so a converged
-// solve returns the plain solver's root up to the tolerance.
+// Optional warm start of the first slip-rate solve of each time point, before
+// its first residual test: a point whose closed-form creep root V_c (the root
+// of g(V) + V = 0, so g(V_c) = -V_c) lies below half the tolerance starts at
+// V_c and passes that test at once; with a 6-32-32-1 MLP (FP32) every other
+// point starts at the network's predicted root when it lies more than a decade
+// from the previous slip rate.  No evaluation is added and the residual test
+// is unchanged; g is strictly decreasing in V, so a converged solve returns
+// the plain solver's root up to the tolerance.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -31,6 +32,15 @@ constexpr double kAlmostZero = 1e-45;
 constexpr int kMaxOrder = 8;
 constexpr double kMovedDecades = 0.30103;  // log10(2)
 constexpr double kJumpDecades = 1.0;
+constexpr double kLn10 = 2.30258509299404568402;
+
+// natural log of the closed-form creep root V_c = 2 V0 exp(-psi/a) sinh(S/N),
+// which solves N asinh(V exp(psi/a) / (2 V0)) = S; -inf when S or N is not positive
+__device__ inline double creepLogRoot(double S, double N, double psiOverA, double sr0) {
+  if (!(S > 0.0) || !(N > 0.0)) return -INFINITY;
+  const double c = S / N;
+  return log(2.0 * sr0) - psiOverA + c + log(-0.5 * expm1(-2.0 * c));
+}
 
 __device__ inline double arsinhexp(double x, double expLog, double ex) {
   const double sw = 10.0, threshold = 50.0, log2 = 0.69314718055994530943;
@@ -154,7 +164,8 @@ __global__ __launch_bounds__(256) void kFriction(sdg_friction_params prm, sdg_fa
   double srm = L.slip_rate_magnitude[k];
   double muv = L.mu[k];
   uint8_t convInner = L.convergence_inner[k], convOuter = L.convergence_outer[k];
-  int newtonEvals = 0;
+  int newtonEvals = 0, newtonSteps = 0;
+  const bool closedForm = opt.closed_form_start != 0;
 
   double updateTime = fullUpdateTime;
   for (int o = 0; o < O; ++o) {
@@ -176,11 +187,27 @@ __global__ __launch_bounds__(256) void kFriction(sdg_friction_params prm, sdg_fa
     srm = fmax(kAlmostZero, sqrt(slipRate1 * slipRate1 + slipRate2 * slipRate2));
     const double vPrev = srm;
     const double upper = fmax(kAlmostZero, invEtaS * absShear);  // g(upper) <= 0: the root lies below
+    const double nScale = invEtaS * fabs(normal) * pp.a;
     double localSR = srm;
     bool convOuterPre = true;
     for (int j = 0; j < prm.rs_state_variable_updates; ++j) {
       sv = active ? updateStateVariable(pp, prm.rs_sr0, svRef, dt, localSR) : 0.0;
       double test = srm;
+      // warm start of the first solve; lvc, base, feat also feed the sample recording
+      double lvc = -INFINITY, base = 0.0;
+      float feat[kMlpInputs];
+      if (j == 0 && active && (closedForm || useMlp || opt.samples != nullptr)) {
+        lvc = creepLogRoot(invEtaS * absShear, nScale, sv / pp.a, prm.rs_sr0);
+        const bool creeping = lvc < log(0.5 * prm.rs_slip_rate_tolerance);
+        if (closedForm && creeping) {
+          test = fmax(kAlmostZero, exp(lvc));
+        } else if (useMlp && !creeping) {
+          base = mlpFeatures(lvc, upper, nScale, sv / pp.a, vPrev, pp.a, feat);
+          // the root lies in (0, min(V_c, S)]
+          const double lg = fmin(base + static_cast<double>(mlpEvaluate(sMlp, feat)), fmin(lvc / kLn10, log10(upper)));
+          if (isfinite(lg) && fabs(lg - log10(vPrev)) > kJumpDecades) test = fmax(kAlmostZero, pow(10.0, lg));
+        }
+      }
       double g = 0.0, muF = muv;
       bool converged = false;
       // Newton on g(V) = -invEtaS (|sigma_n| mu(V) - tau) - V from the
@@ -195,31 +222,19 @@ __global__ __launch_bounds__(256) void kFriction(sdg_friction_params prm, sdg_fa
         if (converged) break;
         if (active) {
           const double dG = -invEtaS * (fabs(normal) * dmuOf(pp.a, prm.rs_sr0, test, sv)) - 1.0;
-          double next = fmax(kAlmostZero, test - g / dG);
-          if (useMlp && j == 0 && it == 0 && !(fabs(g) < prm.rs_slip_rate_tolerance)) {
-            // warm start: the previous slip rate failed the residual test; jump
-            // to the network's root when it predicts a move of more than
-            // kJumpDecades and lies inside the bracket that the sign of g gives
-            float feat[kMlpInputs];
-            const double base = mlpFeatures(upper, invEtaS * fabs(normal) * pp.a, sv / pp.a, vPrev, pp.a,
-                                            prm.rs_sr0, feat);
-            const double lg = base + static_cast<double>(mlpEvaluate(sMlp, feat));
-            const double guess = pow(10.0, lg);
-            const bool inside = g > 0.0 ? (guess > test && guess <= upper) : (guess < test && guess >= kAlmostZero);
-            if (isfinite(guess) && inside && fabs(lg - log10(test)) > kJumpDecades) next = guess;
-          }
-          test = next;
+          test = fmax(kAlmostZero, test - g / dG);
+          ++newtonSteps;
         }
       }
       if (converged) muv = muF;
       else convInner &= (fabs(g) < prm.rs_slip_rate_tolerance) ? 1 : 0;
-      if (opt.samples != nullptr && !useMlp && j == 0 && active && converged) {
-        // (features, log10 root - base) of the first solve of the time point;
-        // solves that move the slip rate by more than a factor of two are all
-        // kept, the others thinned by a hash to 1 in sample_stride
-        float feat[kMlpInputs];
-        const double base = mlpFeatures(upper, invEtaS * fabs(normal) * pp.a, sv / pp.a, vPrev, pp.a,
-                                        prm.rs_sr0, feat);
+      if (opt.samples != nullptr && !useMlp && j == 0 && active && converged &&
+          !(lvc < log(0.5 * prm.rs_slip_rate_tolerance))) {
+        // (features, log10 root - base) of the first solve of the time point
+        // where the closed form does not apply; solves that move the slip rate
+        // by more than a factor of two are all kept, the others thinned by a
+        // hash to 1 in sample_stride
+        base = mlpFeatures(lvc, upper, nScale, sv / pp.a, vPrev, pp.a, feat);
         const uint64_t h = (static_cast<uint64_t>(face) * 2654435761ULL) ^ (static_cast<uint64_t>(p) * 40503ULL) ^
                            (static_cast<uint64_t>(o) * 97ULL) ^ static_cast<uint64_t>(fullUpdateTime * 1e9);
         const bool moved = fabs(log10(test) - log10(vPrev)) > kMovedDecades;
@@ -283,6 +298,7 @@ __global__ __launch_bounds__(256) void kFriction(sdg_friction_params prm, sdg_fa
     L.convergence_inner[k] = convInner;
     L.convergence_outer[k] = convOuter;
     if (L.newton_iterations != nullptr) L.newton_iterations[k] += newtonEvals;
+    if (L.slip_rate_updates != nullptr) L.slip_rate_updates[k] += newtonSteps;
     // imposed states (FrictionSolverCommon.h:333-401)
     double m[9] = {0}, pl[9] = {0};
     for (int o = 0; o < O; ++o) {

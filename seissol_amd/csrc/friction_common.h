@@ -5,13 +5,13 @@
 // The network predicts the root of the first inner slip-rate solve of a time
 // point, g(V) = S - N asinh(V exp(psi / a) / (2 V0)) - V = 0 with
 // S = invEtaS tau (the root lies in (0, S]), N = invEtaS |sigma_n| a and psi
-// the state after the first state update.  Where the asinh argument is large
-// and V << S the root is V = V0 exp(S / N - psi / a); the network learns the
-// correction to
-//   base = min(log10 V0 + (S / N - psi / a) / ln 10, log10 S)
+// the state after the first state update.  The closed-form creep root
+// V_c = 2 V0 exp(-psi / a) sinh(S / N) solves g(V) + V = 0 and bounds the root
+// from above; it is the root itself up to V_c where V_c is small.  The network
+// serves the points where it is not, learning the correction to
+//   base = min(log10 V_c, log10 S)
 // from the features
-//   x = (log10 V0 + (S / N - psi / a) / ln 10, log10 S, log10 N, psi / a,
-//        log10 V_prev, 100 a)
+//   x = (log10 V_c, log10 S, log10 N, psi / a, log10 V_prev, 100 a)
 // and outputs log10(root) - base.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -33,12 +33,11 @@ constexpr int kOffW3 = kOffB2 + kMlpHidden;
 constexpr int kOffB3 = kOffW3 + kMlpHidden;
 constexpr int kMlpParams = kOffB3 + 1;
 
-// features of one solve; returns base (FP64)
-__device__ inline double mlpFeatures(double S, double N, double psiOverA, double vPrev, double a, double sr0,
+// features of one solve from the natural log of V_c; returns base (FP64)
+__device__ inline double mlpFeatures(double logVc, double S, double N, double psiOverA, double vPrev, double a,
                                      float (&x)[kMlpInputs]) {
   const double ls = log10(fmax(S, 1e-300));
-  const double creep = fmin(fmax(log10(sr0) + (S / fmax(N, 1e-300) - psiOverA) * 0.43429448190325182765, -300.0),
-                            300.0);
+  const double creep = fmin(fmax(logVc * 0.43429448190325182765, -300.0), 300.0);
   x[0] = static_cast<float>(creep);
   x[1] = static_cast<float>(ls);
   x[2] = static_cast<float>(log10(fmax(N, 1e-300)));

@@ -144,6 +144,7 @@ def make_fault_setup(num_faces: int = 20_000, order: int = 4, seed: int = 7, poi
     for k in ("rupture_time_pending", "dyn_stress_time_pending", "convergence_inner", "convergence_outer"):
         A[k] = np.ones(shape, dtype=np.uint8)
     A["newton_iterations"] = np.zeros(shape, dtype=np.int32)
+    A["slip_rate_updates"] = np.zeros(shape, dtype=np.int32)
     zp, zs = rho * vp, rho * vs
     ie = np.array([zp, zs, zp, zs, 1 / (1 / zp + 1 / zp), 1 / (1 / zs + 1 / zs), 1 / zs + 1 / zs, 1 / zp, 1 / zs,
                    1 / zp, 1 / zs])
@@ -185,8 +186,10 @@ class DeviceFault:
 class FrictionSolver:
     """Device friction solver for law 103 (FrictionSolver.h:38-65).
 
-    ``mlp_params`` (device FP32, ``mlp_param_count()`` values) switches on the
-    warm start of the first slip-rate solve of each time point."""
+    Warm start of the first slip-rate solve of each time point:
+    ``closed_form_start`` starts creeping points at the closed-form creep root,
+    ``mlp_params`` (device FP32, ``mlp_param_count()`` values) starts the other
+    points at the network's predicted root."""
 
     def __init__(self, params: FrictionParameters, order: int, resample_order: int | None = None):
         import torch
@@ -196,6 +199,7 @@ class FrictionSolver:
         self.resample = torch.from_numpy(R).cuda()
         self.layer = None
         self.mlp_params = None
+        self.closed_form_start = False
 
     @staticmethod
     def computeDeltaT(time_points):
@@ -221,6 +225,8 @@ class FrictionSolver:
         opt = FrictionOptions() if options is None else FrictionOptions.from_buffer_copy(options)
         if self.mlp_params is not None and not opt.mlp_params:
             opt.mlp_params = ptr(self.mlp_params)
+        if self.closed_form_start:
+            opt.closed_form_start = 1
         s = torch.cuda.current_stream().cuda_stream if stream is None else stream
         prm = self.params.c_struct()
         check(lib().sdg_friction_evaluate(C.byref(prm), C.byref(self.layer.view), float(fullUpdateTime),

@@ -200,8 +200,13 @@ class FrictionSolver {
     return dt;
   }
   void setupLayer(const sdg_fault_layer& layer) { layer_ = layer; }
-  /// warm start of the first slip-rate solve of each time point (nullptr: plain Newton)
-  void setWarmStart(const float* deviceMlpParams) { options_.mlp_params = deviceMlpParams; }
+  /// warm start of the first slip-rate solve of each time point: creeping
+  /// points at the closed-form creep root, the others at the MLP's predicted
+  /// root (nullptr: no network); (false, nullptr) is plain Newton
+  void setWarmStart(bool closedForm, const float* deviceMlpParams) {
+    options_.closed_form_start = closedForm ? 1 : 0;
+    options_.mlp_params = deviceMlpParams;
+  }
   void setSampleRecording(float* deviceSamples, std::int64_t capacity, unsigned long long* deviceCount,
                           int stride) {
     options_.samples = deviceSamples;
