@@ -239,6 +239,17 @@ int sdg_lts_destroy(sdg_lts* lts);
  * time-step width dt; the DR kernels run through sdg_proxy_run_dr. */
 int sdg_proxy_kernel_id(const char* name);
 int sdg_proxy_run(sdg_engine* engine, int kernel_id, int steps, double dt, const sdg_layer* layer, void* stream);
+/* No counterpart in the reference.  sdg_proxy_run without any device
+ * allocation or caller workspace, for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops): the matrix-core forms' face traces
+ * live in one library-owned 64 MiB array per GPU (e.g. 19,418 elements at
+ * ORDER 6) and layer->face_traces is ignored; larger layers are rejected.
+ * Kernels 0, 1, 3, 4 (kernel 2 alone is rejected for matrix-core engines: it
+ * would read the traces of an earlier call); the FP64/FP32 engines use no
+ * scratch and run as sdg_proxy_run.  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU race. */
+int sdg_proxy_run_static(sdg_engine* engine, int kernel_id, int steps, double dt, const sdg_layer* layer,
+                         void* stream);
 
 /* Work of one element-update of a proxy kernel (PerformanceEstimate,
  * src/Monitoring/Metric.h:16-50, summed by src/Proxy/Runner.cpp:120-150),

@@ -139,6 +139,7 @@ SIGNATURES = {
     "sdg_compute_batched_ader_local": (I32, [P, P, D, C.POINTER(Layer), P]),
     "sdg_proxy_kernel_id": (I32, [C.c_char_p]),
     "sdg_proxy_run": (I32, [P, I32, I32, D, C.POINTER(Layer), P]),
+    "sdg_proxy_run_static": (I32, [P, I32, I32, D, C.POINTER(Layer), P]),
     "sdg_proxy_dr_faces": (I32, [I64, C.c_uint64, P, P, P, P]),
     "sdg_proxy_run_dr": (I32, [P, I32, I32, D, C.POINTER(Layer), C.POINTER(ProxyDr), P]),
     "sdg_upload_reals": (I32, [P, P, P, I64, P]),

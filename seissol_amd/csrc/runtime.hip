@@ -6,6 +6,7 @@
 #include <cmath>
 #include <functional>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -295,6 +296,36 @@ __global__ void kConvert(int64_t n, const double* __restrict__ src, Dst* __restr
 
 }  // namespace
 
+// Face traces of sdg_proxy_run_static: one array per GPU, held until the
+// process ends.  External linkage keeps it in the code object; host code uses
+// only the per-device address from hipGetSymbolAddress, never the name.
+constexpr int64_t kScratchFloats = int64_t(16) << 20;  // 64 MiB
+__device__ __align__(256) float scratch[kScratchFloats];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex gScratchMutex;
+void* gScratchAddress[kMaxScratchDevices] = {};
+}  // namespace
+
+// device address of the current GPU's scratch array, looked up once per device
+int scratchAddress(void** out) {
+  int dev = 0;
+  SDG_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices) {
+    setError("scratch array: device index beyond " + std::to_string(kMaxScratchDevices));
+    return SDG_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  if (gScratchAddress[dev] == nullptr) {
+    void* p = nullptr;
+    SDG_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    gScratchAddress[dev] = p;
+  }
+  *out = gScratchAddress[dev];
+  return SDG_OK;
+}
+
 // the kernels of proxy kernels 0-4 (validated layer); no allocation
 int proxyRun(const EngineState& st, int kernel_id, int steps, double dt, const sdg_layer* layer, hipStream_t s) {
   double coeffs[kMaxOrder];
@@ -464,6 +495,9 @@ int sdg_engine_create(sdg_engine** engine, int order, int arithmetic) {
     setError("sdg_engine_create: unknown arithmetic");
     return SDG_ERR_INVALID;
   }
+  // the current GPU's scratch address (sdg_proxy_run_static) is resolved at set-up
+  void* scratchPtr = nullptr;
+  if (const int src = scratchAddress(&scratchPtr); src != SDG_OK) return src;
   auto* e = new sdg_engine();
   e->s.order = order;
   e->s.arithmetic = arithmetic;
@@ -598,6 +632,42 @@ int sdg_proxy_run(sdg_engine* engine, int kernel_id, int steps, double dt, const
     return SDG_ERR_INVALID;
   }
   return proxyRun(engine->s, kernel_id, steps, dt, layer, static_cast<hipStream_t>(stream));
+}
+
+int sdg_proxy_run_static(sdg_engine* engine, int kernel_id, int steps, double dt, const sdg_layer* layer,
+                         void* stream) {
+  if (engine == nullptr || layer == nullptr || steps < 0 || kernel_id < 0 || kernel_id > 4) {
+    setError("sdg_proxy_run_static: invalid arguments (kernels 0-4)");
+    return SDG_ERR_INVALID;
+  }
+  if (!matrixCore(engine->s.arithmetic)) return sdg_proxy_run(engine, kernel_id, steps, dt, layer, stream);
+  if (kernel_id == 2) {
+    setError("sdg_proxy_run_static: the neighbour kernel alone would read the face traces of an earlier call");
+    return SDG_ERR_UNSUPPORTED;
+  }
+  int b2p = 0;
+  switch (engine->s.order) {
+    case 3: b2p = mfma::MShape<3>::B2P; break;
+    case 4: b2p = mfma::MShape<4>::B2P; break;
+    case 5: b2p = mfma::MShape<5>::B2P; break;
+    case 6: b2p = mfma::MShape<6>::B2P; break;
+    case 7: b2p = mfma::MShape<7>::B2P; break;
+    default: setError("sdg_proxy_run_static: matrix-core path supports orders 3..7"); return SDG_ERR_UNSUPPORTED;
+  }
+  const int64_t perElement = 36LL * b2p;
+  if (layer->num_elements < 0 || layer->num_elements > kScratchFloats / perElement) {
+    setError("sdg_proxy_run_static: at most " + std::to_string(kScratchFloats / perElement) +
+             " elements at this order (64 MiB of face traces)");
+    return SDG_ERR_INVALID;
+  }
+  // the local pass of every step writes each trace the neighbour pass reads
+  // (padding rows included), so the array needs no clearing between calls
+  void* traces = nullptr;
+  if (const int rc = scratchAddress(&traces); rc != SDG_OK) return rc;
+  sdg_layer L = *layer;
+  L.face_traces = traces;
+  if (!validLayer(engine->s, &L, kernel_id == 0)) return SDG_ERR_INVALID;
+  return proxyRun(engine->s, kernel_id, steps, dt, &L, static_cast<hipStream_t>(stream));
 }
 
 int sdg_upload_reals(sdg_engine* engine, void* device_dst, const double* host_src, int64_t count, void* stream) {
