@@ -187,7 +187,9 @@ int sdg_compute_batched_ader_local(sdg_engine* engine, const double* coeffs, dou
                                    const sdg_layer* layer, void* stream);
 
 /* ---------------------------------------------------------------------- */
-/* Local time stepping (rate 2).  FP64 and FP32 engines.                  */
+/* Local time stepping (rate 2).  Every engine: the matrix-core engines    */
+/* (orders 3-7) run the local pass on the matrix cores and the neighbour    */
+/* pass on the FP32 VALU kernel.                                            */
 /* ---------------------------------------------------------------------- */
 /* deriveLtsSetups (src/Initializer/InitProcedure/Internal/LtsSetup.cpp:97-330,
  * getLtsSetup + normalizeLtsSetup), host: LTS bits of every cell from the
@@ -226,8 +228,10 @@ int sdg_lts_graph_advance(sdg_lts* lts, int macro_steps, void* stream);
  * cluster's data, keeps the fastest and restores the cells' state.
  * Allocates temporaries for the saved state; synchronises `stream`. */
 int sdg_lts_autotune(sdg_lts* lts, int repeats, void* stream);
-/* Tile variant in use for `cluster` and kernel 0 (local) / 1 (neighbour):
- * bit 0 = one more basis row per lane, bit 1 = two waves per block. */
+/* Tile variant in use for `cluster` and kernel 0 (local) / 1 (neighbour).
+ * VALU variants: bit 0 = one more basis row per lane, bit 1 = two waves per
+ * block.  Matrix-core engines, local kernel: 0 = the matrix-core pass,
+ * 1 + v = the FP32 VALU variant v. */
 int sdg_lts_tile_variant(const sdg_lts* lts, int cluster, int kernel);
 int sdg_lts_set_tile_variant(sdg_lts* lts, int cluster, int kernel, int variant);
 int sdg_lts_destroy(sdg_lts* lts);

@@ -104,4 +104,8 @@ int64_t mfmaFragmentCount(int order);
 int launchLocalMfma(const EngineState& e, const double* coeffs, const sdg_layer* layer, bool writeIntegrated,
                     hipStream_t stream);
 int launchNeighborMfma(const EngineState& e, const sdg_layer* layer, hipStream_t stream);
+// the LTS local pass on the matrix cores: buffers and derivatives per the
+// layer's lts_setup (TimeCluster::computeLocalIntegration), no face traces
+int launchLocalIntegrationMfma(const EngineState& e, const double* coeffs, const sdg_layer* layer, int resetBuffers,
+                               hipStream_t stream);
 }  // namespace sdg

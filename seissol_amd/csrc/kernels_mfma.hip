@@ -92,6 +92,18 @@ struct CoeffsF {
   float c[O];
 };
 
+// local-time-stepping outputs of the local pass (TimeCluster.cpp:380-434):
+// per cell its LtsSetup bits say whether it stores its buffer (accumulated
+// over the sub-steps a slower neighbour sees unless `reset`) and its
+// derivatives; setup == nullptr: buffer (if any) stored, derivatives (if any)
+// stored for every cell
+struct LtsOut {
+  const uint16_t* setup = nullptr;
+  float* deriv = nullptr;
+  int reset = 1;
+  int pad_ = 0;
+};
+
 // geometry of the two kernels per order and variant
 template <int O, int V>
 struct Cfg {
@@ -408,12 +420,12 @@ __device__ inline void gemmStars(const uint16_t* const (&a)[3], int ks, const ty
   }
 }
 
-template <int O, int V, bool V4>
+template <int O, int V, bool V4, bool LTS>
 __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLocalMfma(
     int64_t n, const uint16_t* __restrict__ frag, const float* __restrict__ tables, CoeffsF<O> cf,
     float* __restrict__ dofs, float* __restrict__ integ, float* __restrict__ traces,
     const float* __restrict__ star, const float* __restrict__ aplus, const uint8_t* __restrict__ faceTypes,
-    CellLayout lay) {
+    CellLayout lay, LtsOut lts) {
   using S = MShape<O>;
   using C = Cfg<O, V>;
   using T = typename Prec<V>::T;
@@ -464,6 +476,22 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
   // Q itself is read again at the end, so only the time integral stays live
   float q[1][9][4], iacc[1][9][4];
   loadRows<B, V4>(tileDofs + laneQ, ldq, 16 * w + 4 * g, valid, q[0]);
+  // LTS: which of buffer and derivatives this cell provides
+  [[maybe_unused]] unsigned setup = 0;
+  [[maybe_unused]] bool storeDeriv = false;
+  [[maybe_unused]] float* cellDeriv = nullptr;
+  if constexpr (LTS) {
+    setup = (lts.setup != nullptr && valid) ? lts.setup[elem] : 0u;
+    storeDeriv = lts.deriv != nullptr && valid && (lts.setup == nullptr || (setup & SDG_LTS_HAS_DERIVATIVES));
+    cellDeriv = lts.deriv + (valid ? elem : 0) * lay.dsize;
+    if (storeDeriv)
+#pragma unroll
+      for (int p = 0; p < 9; ++p)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (16 * w + 4 * g + j < B) cellDeriv[p * ldq + 16 * w + 4 * g + j] = q[0][p][j];
+  }
+  [[maybe_unused]] int64_t doff = 9LL * ldq;
 #pragma unroll
   for (int p = 0; p < 9; ++p)
 #pragma unroll
@@ -545,6 +573,17 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
 #pragma unroll
           for (int p = 0; p < 9; ++p) dq[0][p][j] = 0.0f;
     }
+    if constexpr (LTS) {
+      // derivative block d: [9][rows(m)] after the blocks before it
+      const int ldm = lay.rows(m);
+      if (storeDeriv && w < mtiles)
+#pragma unroll
+        for (int p = 0; p < 9; ++p)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (16 * w + 4 * g + j < m) cellDeriv[doff + p * ldm + 16 * w + 4 * g + j] = dq[0][p][j];
+      doff += 9LL * ldm;
+    }
     if constexpr (scaled) waveColumnMax<1>(dq, w < mtiles, sMax, w, lane);
     __syncthreads();
     if constexpr (scaled) columnScales<MT>(sMax, e, cs, un);
@@ -562,15 +601,26 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     __syncthreads();
   }
 
-  // time-integrated DOFs: optional output, and the operand of the next GEMMs
-  if (integ != nullptr && valid) {
+  // time-integrated DOFs: optional output (LTS: the buffer, stored or
+  // accumulated), and the operand of the next GEMMs
+  bool storeBuffer = integ != nullptr && valid;
+  [[maybe_unused]] bool accumulate = false;
+  if constexpr (LTS) {
+    storeBuffer = storeBuffer && (lts.setup == nullptr || (setup & SDG_LTS_HAS_BUFFERS));
+    accumulate = lts.setup != nullptr && !lts.reset && (setup & SDG_LTS_ACCUMULATE_BUFFERS);
+  }
+  if (storeBuffer) {
     float* const tileInteg = integ + E0 * lay.qStride;
 #pragma unroll
     for (int p = 0; p < 9; ++p)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int row = 16 * w + 4 * g + j;
-        if (row < B) tileInteg[laneQ + p * ldq + j] = iacc[0][p][j];
+        if (row < B) {
+          float* b = tileInteg + laneQ + p * ldq + j;
+          if constexpr (LTS) *b = accumulate ? *b + iacc[0][p][j] : iacc[0][p][j];
+          else *b = iacc[0][p][j];
+        }
       }
   }
   if constexpr (scaled) {
@@ -608,7 +658,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
       // face (faces padded to B2P rows, a multiple of 4), one 16-byte store
       // per quantity into the padded layout [element][face][quantity][B2P]
       const int r0 = 16 * t + 4 * g;
-      if (valid && r0 < FR) {
+      if (valid && r0 < FR && (!LTS || traces != nullptr)) {
         const int f = r0 / S::B2P, mm = r0 % S::B2P;
 #pragma unroll
         for (int p = 0; p < 9; ++p)
@@ -928,9 +978,12 @@ inline bool vectorAccess(const CellLayout& lay, const void* dofs, const void* in
   return true;
 }
 
+// lts == nullptr: the proxy/GTS local pass (traces written, derivatives never);
+// otherwise the LTS local pass: buffers and derivatives per the cells' setup,
+// traces written only when the layer has them
 template <int O, int V>
 int localImpl(const EngineState& st, const double* coeffs, const sdg_layer* L, bool writeIntegrated,
-              hipStream_t stream) {
+              hipStream_t stream, const LtsOut* lts = nullptr) {
   using S = MShape<O>;
   const int64_t n = L->num_elements;
   if (n == 0) return SDG_OK;
@@ -942,10 +995,16 @@ int localImpl(const EngineState& st, const double* coeffs, const sdg_layer* L, b
   if (!vectorAccess(lay, L->dofs, integ, "matrix-core local pass", v4)) return SDG_ERR_INVALID;
   const dim3 grid(static_cast<unsigned>((n + kTileElems - 1) / kTileElems));
   auto launch = [&](auto vec) {
-    kLocalMfma<O, V, decltype(vec)::value><<<grid, S::THREADS, 0, stream>>>(
-        n, static_cast<const uint16_t*>(st.mfmaFrags), static_cast<const float*>(st.mfmaTables), cf,
-        static_cast<float*>(L->dofs), integ, static_cast<float*>(L->face_traces), static_cast<const float*>(L->star),
-        static_cast<const float*>(L->aplus), L->face_types, lay);
+    if (lts != nullptr)
+      kLocalMfma<O, V, decltype(vec)::value, true><<<grid, S::THREADS, 0, stream>>>(
+          n, static_cast<const uint16_t*>(st.mfmaFrags), static_cast<const float*>(st.mfmaTables), cf,
+          static_cast<float*>(L->dofs), integ, static_cast<float*>(L->face_traces),
+          static_cast<const float*>(L->star), static_cast<const float*>(L->aplus), L->face_types, lay, *lts);
+    else
+      kLocalMfma<O, V, decltype(vec)::value, false><<<grid, S::THREADS, 0, stream>>>(
+          n, static_cast<const uint16_t*>(st.mfmaFrags), static_cast<const float*>(st.mfmaTables), cf,
+          static_cast<float*>(L->dofs), integ, static_cast<float*>(L->face_traces),
+          static_cast<const float*>(L->star), static_cast<const float*>(L->aplus), L->face_types, lay, LtsOut{});
   };
   if constexpr (S::B % 4 != 0) {
     if (!v4) launch(std::false_type{});
@@ -984,11 +1043,11 @@ int neighborImpl(const EngineState& st, const sdg_layer* L, hipStream_t stream) 
 
 template <int O>
 int localDispatch(const EngineState& e, const double* coeffs, const sdg_layer* layer, bool writeIntegrated,
-                  hipStream_t stream) {
+                  hipStream_t stream, const LtsOut* lts) {
   switch (e.arithmetic) {
-    case SDG_BF16X6: return localImpl<O, 3>(e, coeffs, layer, writeIntegrated, stream);
-    case SDG_F16X3: return localImpl<O, 16>(e, coeffs, layer, writeIntegrated, stream);
-    default: return localImpl<O, 2>(e, coeffs, layer, writeIntegrated, stream);
+    case SDG_BF16X6: return localImpl<O, 3>(e, coeffs, layer, writeIntegrated, stream, lts);
+    case SDG_F16X3: return localImpl<O, 16>(e, coeffs, layer, writeIntegrated, stream, lts);
+    default: return localImpl<O, 2>(e, coeffs, layer, writeIntegrated, stream, lts);
   }
 }
 
@@ -1015,16 +1074,34 @@ int64_t mfmaFragmentCount(int order) {
   }
 }
 
-int launchLocalMfma(const EngineState& e, const double* coeffs, const sdg_layer* layer, bool writeIntegrated,
-                    hipStream_t stream) {
+namespace {
+int localOrders(const EngineState& e, const double* coeffs, const sdg_layer* layer, bool writeIntegrated,
+                hipStream_t stream, const mfma::LtsOut* lts) {
   switch (e.order) {
-    case 3: return mfma::localDispatch<3>(e, coeffs, layer, writeIntegrated, stream);
-    case 4: return mfma::localDispatch<4>(e, coeffs, layer, writeIntegrated, stream);
-    case 5: return mfma::localDispatch<5>(e, coeffs, layer, writeIntegrated, stream);
-    case 6: return mfma::localDispatch<6>(e, coeffs, layer, writeIntegrated, stream);
-    case 7: return mfma::localDispatch<7>(e, coeffs, layer, writeIntegrated, stream);
+    case 3: return mfma::localDispatch<3>(e, coeffs, layer, writeIntegrated, stream, lts);
+    case 4: return mfma::localDispatch<4>(e, coeffs, layer, writeIntegrated, stream, lts);
+    case 5: return mfma::localDispatch<5>(e, coeffs, layer, writeIntegrated, stream, lts);
+    case 6: return mfma::localDispatch<6>(e, coeffs, layer, writeIntegrated, stream, lts);
+    case 7: return mfma::localDispatch<7>(e, coeffs, layer, writeIntegrated, stream, lts);
     default: setError("matrix-core path supports orders 3..7"); return SDG_ERR_UNSUPPORTED;
   }
+}
+}  // namespace
+
+int launchLocalMfma(const EngineState& e, const double* coeffs, const sdg_layer* layer, bool writeIntegrated,
+                    hipStream_t stream) {
+  if (layer->derivatives != nullptr) {
+    // the fused ader + local pass with derivatives (every cell stores them)
+    const mfma::LtsOut lts{nullptr, static_cast<float*>(layer->derivatives), 1, 0};
+    return localOrders(e, coeffs, layer, writeIntegrated, stream, &lts);
+  }
+  return localOrders(e, coeffs, layer, writeIntegrated, stream, nullptr);
+}
+
+int launchLocalIntegrationMfma(const EngineState& e, const double* coeffs, const sdg_layer* layer, int resetBuffers,
+                               hipStream_t stream) {
+  const mfma::LtsOut lts{layer->lts_setup, static_cast<float*>(layer->derivatives), resetBuffers, 0};
+  return localOrders(e, coeffs, layer, layer->time_integrated != nullptr, stream, &lts);
 }
 
 int launchNeighborMfma(const EngineState& e, const sdg_layer* layer, hipStream_t stream) {

@@ -53,9 +53,21 @@ bool ltsLayerOk(const sdg_layer* L, bool neighbors, const char* who) {
   return true;
 }
 
-bool vectorArithmetic(const sdg_engine* e, const char* who) {
-  if (e->s.arithmetic != SDG_FP64 && e->s.arithmetic != SDG_FP32) {
-    setError(std::string(who) + ": local time stepping runs on the FP64 and FP32 engines");
+// Local-pass variants: the FP64/FP32 engines have the kTileVariants VALU tile
+// shapes; a matrix-core engine has its matrix-core pass (variant 0) and the
+// FP32 VALU shapes after it.  The neighbour pass is the VALU kernel of the
+// engine's real type (FP32 for the matrix-core engines) with kTileVariants shapes.
+int localVariants(const EngineState& e) { return matrixCore(e.arithmetic) ? 1 + kTileVariants : kTileVariants; }
+
+int runLocal(const EngineState& e, const double* coeffs, const sdg_layer* own, int reset, hipStream_t s, int v) {
+  if (!matrixCore(e.arithmetic)) return launchLocalIntegration(e, coeffs, own, reset, s, v);
+  if (v == 0) return launchLocalIntegrationMfma(e, coeffs, own, reset, s);
+  return launchLocalIntegration(e, coeffs, own, reset, s, v - 1);
+}
+
+bool ltsArithmetic(const sdg_engine* e, const char* who) {
+  if (matrixCore(e->s.arithmetic) && (e->s.order < 3 || e->s.order > 7)) {
+    setError(std::string(who) + ": the matrix-core forms run orders 3..7");
     return false;
   }
   return true;
@@ -112,11 +124,11 @@ extern "C" int sdg_compute_local_integration(sdg_engine* engine, const double* c
     setError("sdg_compute_local_integration: NULL argument");
     return SDG_ERR_INVALID;
   }
-  if (!vectorArithmetic(engine, "sdg_compute_local_integration") ||
+  if (!ltsArithmetic(engine, "sdg_compute_local_integration") ||
       !ltsLayerOk(layer, false, "sdg_compute_local_integration") ||
       !layerStridesOk(engine->s, layer, "sdg_compute_local_integration"))
     return SDG_ERR_INVALID;
-  return launchLocalIntegration(engine->s, coeffs, layer, reset_buffers != 0, static_cast<hipStream_t>(stream));
+  return runLocal(engine->s, coeffs, layer, reset_buffers != 0, static_cast<hipStream_t>(stream), 0);
 }
 
 extern "C" int sdg_compute_neighboring_integration(sdg_engine* engine, const double* time_coeffs,
@@ -126,7 +138,7 @@ extern "C" int sdg_compute_neighboring_integration(sdg_engine* engine, const dou
     setError("sdg_compute_neighboring_integration: NULL argument");
     return SDG_ERR_INVALID;
   }
-  if (!vectorArithmetic(engine, "sdg_compute_neighboring_integration") ||
+  if (!ltsArithmetic(engine, "sdg_compute_neighboring_integration") ||
       !ltsLayerOk(layer, true, "sdg_compute_neighboring_integration") ||
       !layerStridesOk(engine->s, layer, "sdg_compute_neighboring_integration"))
     return SDG_ERR_INVALID;
@@ -186,7 +198,7 @@ int stepCluster(const sdg_lts& L, int c, int64_t k, hipStream_t s) {
   const sdg_layer own = clusterView(L, c, false);
   int rc = SDG_OK;
   if (own.num_elements > 0)
-    rc = launchLocalIntegration(L.engine->s, timeCoeffs.data(), &own, (stepIndex & 1) == 0, s, L.tileLocal[c]);
+    rc = runLocal(L.engine->s, timeCoeffs.data(), &own, (stepIndex & 1) == 0, s, L.tileLocal[c]);
   if (rc != SDG_OK) return rc;
   if (c > 0) {
     if ((rc = stepCluster(L, c - 1, k, s)) != SDG_OK) return rc;
@@ -219,7 +231,7 @@ extern "C" int sdg_lts_create(sdg_lts** out, sdg_engine* engine, const sdg_layer
     setError("sdg_lts_create: invalid arguments (1..16 clusters, dt0 > 0)");
     return SDG_ERR_INVALID;
   }
-  if (!vectorArithmetic(engine, "sdg_lts_create") || !ltsLayerOk(layer, true, "sdg_lts_create") ||
+  if (!ltsArithmetic(engine, "sdg_lts_create") || !ltsLayerOk(layer, true, "sdg_lts_create") ||
       !layerStridesOk(engine->s, layer, "sdg_lts_create"))
     return SDG_ERR_INVALID;
   if (cluster_start[0] != 0 || cluster_start[num_clusters] != layer->num_elements) {
@@ -286,10 +298,11 @@ extern "C" int sdg_lts_autotune(sdg_lts* lts, int repeats, void* stream) {
     for (int kind = 0; kind < 2 && rc == SDG_OK; ++kind) {
       float best = 0.0f;
       int bestV = 0;
-      for (int v = 0; v < kTileVariants && rc == SDG_OK; ++v) {
+      const int variants = kind == 0 ? localVariants(lts->engine->s) : kTileVariants;
+      for (int v = 0; v < variants && rc == SDG_OK; ++v) {
         (void)hipEventRecord(e0, st);
         for (int r = 0; r < repeats && rc == SDG_OK; ++r)
-          rc = kind == 0 ? launchLocalIntegration(lts->engine->s, coeffs.data(), &own, 1, st, v)
+          rc = kind == 0 ? runLocal(lts->engine->s, coeffs.data(), &own, 1, st, v)
                          : launchNeighboringIntegration(lts->engine->s, coeffs.data(), coeffs.data(), &nbv, st, v);
         (void)hipEventRecord(e1, st);
         if (rc != SDG_OK) break;
@@ -333,7 +346,7 @@ extern "C" int sdg_lts_autotune(sdg_lts* lts, int repeats, void* stream) {
 
 extern "C" int sdg_lts_set_tile_variant(sdg_lts* lts, int cluster, int kernel, int variant) {
   if (lts == nullptr || cluster < 0 || cluster >= static_cast<int>(lts->tileLocal.size()) || kernel < 0 ||
-      kernel > 1 || variant < 0 || variant >= kTileVariants) {
+      kernel > 1 || variant < 0 || variant >= (kernel == 0 ? localVariants(lts->engine->s) : kTileVariants)) {
     setError("sdg_lts_set_tile_variant: invalid arguments");
     return SDG_ERR_INVALID;
   }

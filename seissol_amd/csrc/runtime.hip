@@ -575,14 +575,8 @@ int sdg_compute_batched_ader_local(sdg_engine* engine, const double* coeffs, dou
     setError("star/aplus matrices missing");
     return SDG_ERR_INVALID;
   }
-  if (matrixCore(engine->s.arithmetic) && layer->face_traces != nullptr) {
-    if (layer->derivatives != nullptr) {
-      setError("sdg_compute_batched_ader_local: the matrix-core local pass writes face traces, not derivatives; "
-               "pass derivatives = NULL (or face_traces = NULL for the VALU pass that writes them)");
-      return SDG_ERR_UNSUPPORTED;
-    }
+  if (matrixCore(engine->s.arithmetic) && layer->face_traces != nullptr)
     return launchLocalMfma(engine->s, coeffs, layer, true, static_cast<hipStream_t>(stream));
-  }
   return launchAderLocal(engine->s, coeffs, layer, static_cast<hipStream_t>(stream));
 }
 

@@ -103,12 +103,64 @@ def test_lts_graph_replay_equals_eager(gpu):
         del scratch
 
 
-def test_lts_rejects_matrix_core_engines_and_missing_setup(gpu):
-    lts = make_lts_setup(3, fractions=(0.5, 0.5), dims=(6, 4, 4), seed=1)
-    eng = Engine(3, "f16x3")
+@pytest.mark.parametrize("order,fractions,arith,tol", [
+    (5, (0.6, 0.25, 0.1, 0.05), "f16x3", 1e-5),
+    (3, (0.4, 0.35, 0.25), "f16x3", 1e-5),
+    (4, (0.5, 0.3, 0.2), "bf16x3", 5e-5),
+])
+def test_lts_on_the_matrix_cores_matches_oracle(gpu, order, fractions, arith, tol):
+    # local pass on the matrix cores (buffers and derivatives per the LTS
+    # bits), neighbour pass on the FP32 VALU kernel
+    import torch
+    dims = (20, 6, 6) if len(fractions) == 4 else (10, 6, 6)
+    lts = make_lts_setup(order, fractions=fractions, dims=dims, seed=3)
+    print(f"lts order={order} clusters={lts.num_clusters} n={lts.setup.n} {arith}", flush=True)
+    eng = Engine(order, arith)
     layer = DeviceLtsLayer(eng, lts)
+    run = LocalTimeStepping(eng, layer)
+    assert all(t["local"] == 0 for t in run.tiles().values())  # the matrix-core pass by default
+    run.advance(2)
+    torch.cuda.synchronize()
+    q_ref = _oracle_q(lts, order, 2, layer)
+    q = _np(layer.packed(layer.dofs)).astype(np.float64)
+    per_q = [rel_l2(q[:, p], q_ref[:, p]) for p in range(9)]
+    print(f"rel L2 per quantity vs oracle after 2 macro steps: max {max(per_q):.2e}", flush=True)
+    assert max(per_q) < tol
+
+
+def test_matrix_core_lts_variants_and_autotuner(gpu):
+    import torch
+    order = 5
+    lts = make_lts_setup(order, fractions=(0.5, 0.3, 0.2), dims=(10, 6, 6), seed=6)
+    eng = Engine(order, "f16x3")
+    q_ref = None
+    for variant in range(5):  # 0: matrix cores, 1..4: the FP32 VALU tiles
+        layer = DeviceLtsLayer(eng, lts)
+        run = LocalTimeStepping(eng, layer)
+        run.set_tile(variant, kernel=0)
+        run.advance(2)
+        torch.cuda.synchronize()
+        if q_ref is None:
+            q_ref = _oracle_q(lts, order, 2, layer)
+        err = rel_l2(_np(layer.packed(layer.dofs)).astype(np.float64), q_ref)
+        print(f"f16x3 local variant {variant}: rel L2 vs oracle {err:.2e}", flush=True)
+        assert err < 1e-5
+    layer = DeviceLtsLayer(eng, lts)
+    run = LocalTimeStepping(eng, layer)
+    before = layer.dofs.clone()
+    tiles = run.autotune(repeats=2)
+    torch.cuda.synchronize()
+    assert bool(torch.equal(before, layer.dofs))
+    assert all(0 <= t["local"] < 5 and 0 <= t["neigh"] < 4 for t in tiles.values())
     with pytest.raises(_capi.SdgError):
-        LocalTimeStepping(eng, layer)
+        run.set_tile(5, kernel=0)
+    run.advance(2, graph=True)
+    torch.cuda.synchronize()
+    assert rel_l2(_np(layer.packed(layer.dofs)).astype(np.float64), q_ref) < 1e-5
+
+
+def test_lts_rejects_missing_setup(gpu):
+    lts = make_lts_setup(3, fractions=(0.5, 0.5), dims=(6, 4, 4), seed=1)
     eng64 = Engine(3, "fp64")
     layer64 = DeviceLtsLayer(eng64, lts)
     layer64.view.lts_setup = None
