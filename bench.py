@@ -1,10 +1,12 @@
 """Benchmark of the ADER-DG element update (proxy kernel `all`) on one GPU.
 
 Workload (default): elastic ORDER 6 (56 basis functions x 9 quantities),
-1,000,000 tetrahedra of a conforming periodic jittered box (50 x 50 x 80 cubes,
-5 tets each), rho ~ U[2000, 3000], vp 6000, vs 3464, DOFs ~ U[-1, 1], dt from
-CFL 0.5, global time stepping, 200 steps, in the chosen split-fp16 matrix-core
-form; one step = ader + local integral + neighbour integral of every element.
+1,000,000 tetrahedra of the reference proxy's layer (seeded random affine
+tetrahedra, 4 random face neighbours per element with random sides and
+orientations, periodic wrap), star matrices stored as their 24 non-zeros,
+rho ~ U[2000, 3000], vp 6000, vs 3464, DOFs ~ U[-1, 1], dt from CFL 0.5,
+global time stepping, 200 steps, in the chosen split-fp16 matrix-core form;
+one step = ader + local integral + neighbour integral of every element.
 Inputs are resident in HBM before timing starts.
 
 Prints one JSON line (rank 0).  Besides the throughput it reports the plain
@@ -55,13 +57,13 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(order: int, elements: int, steps: int, seed: int) -> dict:
+def cpu_baseline(order: int, elements: int, steps: int, seed: int, mesh: str) -> dict:
     import oracle
     from seissol_amd.kernels import global_matrices, monomial_integrate
     from seissol_amd.proxy import make_setup
 
     oracle.build()
-    setup = make_setup(order, n=elements, mesh="periodic", seed=seed)
+    setup = make_setup(order, n=elements, mesh=mesh, seed=seed)
     gm = global_matrices(order)["flat"]
     star, ap, am = oracle.cell_local_matrices(setup.vertices, setup.material, setup.neighbors)
     coeffs = monomial_integrate(order, 0.0, setup.dt)
@@ -72,15 +74,17 @@ def cpu_baseline(order: int, elements: int, steps: int, seed: int) -> dict:
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": setup.n * steps / sec, "unit": "element-updates/s", "cores": threads, "kind": "port",
             "cpu_model": cpu_model(),
-            "sample": f"ORDER {order}, {setup.n} elements x {steps} steps, proxy `all`, FP64 OpenMP oracle "
+            "sample": f"ORDER {order}, {setup.n} elements ({mesh} mesh) x {steps} steps, proxy `all`, FP64 OpenMP oracle "
                       f"(CPU restatement, not the reference binary), {sec:.1f} s"}
 
 
-def measured_traffic(form: str, order: int, elements: int, kernel: str):
+def measured_traffic(form: str, order: int, elements: int, kernel: str, mesh: str, star: str):
     """HBM bytes per launch of `kernel` from a counter summary under profiles/
-    (a line `bench-traffic: form=F order=O elements=N kernel=K bytes=B`), or None."""
+    (a line `bench-traffic: form=F order=O elements=N mesh=M star=S kernel=K
+    bytes=B`), or None."""
     from pathlib import Path
-    want = {"form": form, "order": str(order), "elements": str(elements), "kernel": kernel}
+    want = {"form": form, "order": str(order), "elements": str(elements), "kernel": kernel, "mesh": mesh,
+            "star": star}
     for path in sorted((Path(__file__).resolve().parent / "profiles").glob("*.md")):
         for line in path.read_text().splitlines():
             if not line.startswith("bench-traffic:"):
@@ -88,6 +92,23 @@ def measured_traffic(form: str, order: int, elements: int, kernel: str):
             fields = dict(kv.split("=", 1) for kv in line.split(":", 1)[1].split())
             if all(fields.get(k) == v for k, v in want.items()):
                 return float(fields["bytes"])
+    return None
+
+
+def measured_mfma(form: str, order: int, elements: int, kernel: str, mesh: str, star: str):
+    """fp16/bf16 matrix-core FLOP per launch and MFMA busy fraction of `kernel`
+    from a counter summary under profiles/ (a line `bench-mfma: form=F order=O
+    elements=N mesh=M star=S kernel=K flop=X busy_frac=Y`), or None."""
+    from pathlib import Path
+    want = {"form": form, "order": str(order), "elements": str(elements), "kernel": kernel, "mesh": mesh,
+            "star": star}
+    for path in sorted((Path(__file__).resolve().parent / "profiles").glob("*.md")):
+        for line in path.read_text().splitlines():
+            if not line.startswith("bench-mfma:"):
+                continue
+            fields = dict(kv.split("=", 1) for kv in line.split(":", 1)[1].split())
+            if all(fields.get(k) == v for k, v in want.items()):
+                return {"flop": float(fields["flop"]), "busy_frac": float(fields["busy_frac"]), "source": path.name}
     return None
 
 
@@ -267,7 +288,11 @@ def main(argv=None) -> int:
     ap.add_argument("--order", type=int, default=6)
     ap.add_argument("--elements", type=int, default=1_000_000)
     ap.add_argument("--form", choices=["fp64", "fp32", "bf16x3", "bf16x6", "f16x3"], default="f16x3")
-    ap.add_argument("--mesh", choices=["periodic", "proxy"], default="periodic")
+    ap.add_argument("--mesh", choices=["periodic", "proxy"], default="proxy",
+                    help="proxy: 4 random face neighbours per element with periodic wrap (the reference "
+                         "proxy's layer); periodic: a conforming periodic box")
+    ap.add_argument("--star", choices=["dense", "compact"], default="compact",
+                    help="star matrices as 3 x 81 dense or 3 x 24 non-zeros (the reference's sparse GPU layout)")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--cpu-elements", type=int, default=100_000)
     ap.add_argument("--cpu-steps", type=int, default=20)
@@ -322,6 +347,7 @@ def main(argv=None) -> int:
     setup = make_setup(args.order, n=args.elements, mesh=args.mesh, seed=args.seed, dims=dims)
     log(f"setup {setup.n} elements ({args.mesh}) in {time.perf_counter() - t0:.1f} s, dt={setup.dt:.3e}")
     eng = Engine(args.order, args.form)
+    eng.set_layout(0, args.star)
     layer = DeviceLayer(eng, setup)
     torch.cuda.synchronize()
     log("device layer ready")
@@ -354,34 +380,70 @@ def main(argv=None) -> int:
     achieved = ab[dom] * setup.n / (per_kernel[dom] / 1e3) / 1e9
     log(f"per launch: local {per_kernel['local']:.3f} ms, neigh {per_kernel['neigh']:.3f} ms")
 
-    # accuracy of the form: same initial state, same steps, against the plain
-    # FP64 path (itself pinned to the CPU oracle by the parity tests), and the
-    # plain FP64 path's own throughput on that run
+    # accuracy of the form against the plain FP64 path (itself pinned to the
+    # CPU oracle by the parity tests), from the same initial state:
+    #  * on the benchmarked mesh, in lockstep checkpoints of `check_every`
+    #    steps; the proxy's random face neighbours make an inconsistent mesh on
+    #    which the solution grows without bound, so the growth (largest |Q| of
+    #    the FP64 run) is reported with the error until it stops being finite;
+    #  * on the conforming periodic box of the same size, after the full step
+    #    count: the 1e-5 gate of the metric.
     error, plain = None, None
     if args.form != "fp64" and not args.no_error_check:
-        layer.reset_dofs(setup.dofs)
-        run_proxy(eng, layer, "all", args.steps, setup.dt)
-        q_form = layer.dofs.double()
+        def per_quantity(a, b):
+            diff = (a.double() - b).pow(2).sum(dim=(0, 2)).sqrt()
+            ref = b.pow(2).sum(dim=(0, 2)).sqrt()
+            return (diff / ref).tolist()
+
+        def lockstep(su, lay_form, eng_form, every):
+            lay_form.reset_dofs(su.dofs)
+            e64 = Engine(args.order, "fp64")
+            e64.set_layout(0, args.star)
+            l64 = DeviceLayer(e64, su)
+            torch.cuda.synchronize()
+            points, ms64, done = [], 0.0, 0
+            while done < args.steps:
+                k = min(every, args.steps - done)
+                run_proxy(eng_form, lay_form, "all", k, su.dt)
+                s64, e64e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                s64.record()
+                run_proxy(e64, l64, "all", k, su.dt)
+                e64e.record()
+                torch.cuda.synchronize()
+                ms64 += s64.elapsed_time(e64e)
+                done += k
+                amax = float(l64.dofs.abs().max().item())
+                fin = bool(torch.isfinite(lay_form.dofs).all().item()) and math.isfinite(amax)
+                pq = per_quantity(lay_form.dofs, l64.dofs) if fin else None
+                points.append({"step": done, "max_abs_q_fp64": amax, "finite": fin,
+                               "rel_l2_max": max(pq) if pq else None})
+                if not fin:
+                    break
+            final = per_quantity(lay_form.dofs, l64.dofs) if points[-1]["finite"] else None
+            del l64
+            return points, final, ms64 / done
+
+        points, final, ms64 = lockstep(setup, layer, eng, 10)
+        plain = {"value": setup.n / (ms64 / 1e3), "ms_per_step": ms64, "form": "fp64 (VALU, the drop-in)",
+                 "mesh": args.mesh}
+        last_ok = [p for p in points if p["finite"]]
+        log(f"plain FP64: {plain['value'] / 1e6:.2f} M element-updates/s; {args.mesh} mesh: finite through step "
+            f"{last_ok[-1]['step'] if last_ok else 0}, rel L2 there {last_ok[-1]['rel_l2_max'] if last_ok else None}")
         del layer
-        eng64 = Engine(args.order, "fp64")
-        layer64 = DeviceLayer(eng64, setup)
-        torch.cuda.synchronize()
-        s64, e64 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s64.record()
-        run_proxy(eng64, layer64, "all", args.steps, setup.dt)
-        e64.record()
-        torch.cuda.synchronize()
-        ms64 = s64.elapsed_time(e64)
-        plain = {"value": setup.n * args.steps / (ms64 / 1e3), "ms_per_step": ms64 / args.steps,
-                 "form": "fp64 (VALU, the drop-in)"}
-        diff = (q_form - layer64.dofs).pow(2).sum(dim=(0, 2)).sqrt()
-        ref = layer64.dofs.pow(2).sum(dim=(0, 2)).sqrt()
-        per_q = (diff / ref).tolist()
-        error = {"steps": args.steps, "rel_l2_per_quantity": per_q, "max": max(per_q),
-                 "reference": "plain FP64 GPU path, same input"}
-        log(f"plain FP64: {plain['value'] / 1e6:.2f} M element-updates/s; rel L2 vs FP64 after {args.steps} steps: "
-            f"max {max(per_q):.2e}")
-        del layer64, q_form
+        torch.cuda.empty_cache()
+        gate_setup = setup
+        if args.mesh != "periodic":
+            gate_setup = make_setup(args.order, n=args.elements, mesh="periodic", seed=args.seed,
+                                    dims=known.get(args.elements))
+        gate_layer = DeviceLayer(eng, gate_setup)
+        gpoints, gfinal, _ = lockstep(gate_setup, gate_layer, eng, args.steps)
+        del gate_layer
+        error = {"steps": args.steps, "rel_l2_per_quantity": gfinal, "max": max(gfinal) if gfinal else None,
+                 "mesh": f"conforming periodic box, {gate_setup.n} tetrahedra",
+                 "reference": "plain FP64 GPU path, same input",
+                 "benchmarked_mesh": {"mesh": args.mesh, "checkpoints": points,
+                                      "rel_l2_per_quantity_at_end": final}}
+        log(f"rel L2 vs FP64 after {args.steps} steps on the periodic box: max {error['max']:.2e}")
 
     # matrix-core work of the dominant kernel (dense products the form executes)
     from seissol_amd._capi import WorkEstimate, check, lib
@@ -391,7 +453,8 @@ def main(argv=None) -> int:
     check(lib().sdg_performance_estimate(args.order, ARITHMETIC[args.form], 1 if dom == "local" else 2,
                                          C.byref(est)), "sdg_performance_estimate")
     mma_tflops = est.hardware_flop * setup.n / (per_kernel[dom] / 1e3) / 1e12
-    traffic = measured_traffic(args.form, args.order, setup.n, dom)
+    traffic = measured_traffic(args.form, args.order, setup.n, dom, args.mesh, args.star)
+    mfma = measured_mfma(args.form, args.order, setup.n, dom, args.mesh, args.star)
 
     result = {
         "metric": METRIC,
@@ -411,20 +474,27 @@ def main(argv=None) -> int:
                   "f16x3": "f32 (2-plane split-fp16 MFMA on power-of-two scaled operands, f32 accumulate)"}[args.form],
         "data": "synthetic",
         "config": {"workload": f"elastic ADER-DG element update (proxy `all`), ORDER {args.order}, "
-                               f"{setup.n} tetrahedra, {args.mesh} mesh, {args.form}, global time stepping",
-                   "order": args.order, "elements": setup.n, "form": args.form, "mesh": args.mesh,
-                   "seed": args.seed, "dt": setup.dt, "finite": finite, "error_vs_fp64": error},
+                               f"{setup.n} tetrahedra, {args.mesh} mesh, {args.form}, {args.star} star, "
+                               f"global time stepping",
+                   "order": args.order, "elements": setup.n, "form": args.form, "mesh": args.mesh, "star": args.star,
+                   "seed": args.seed, "dt": setup.dt, "values_finite_after_timed_steps": finite,
+                   "error_vs_fp64": error},
         "roofline": {"bound": "hbm", "kernel": f"proxy `{dom}` ({'ader+volume+local flux' if dom == 'local' else 'neighbour flux'})",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_element": ab[dom],
                      "launch_ms": per_kernel[dom], "other_launch_ms": {k: v for k, v in per_kernel.items() if k != dom},
-                     "matrix_core": {"achieved_tflops": mma_tflops,
-                                     "peak_tflops": 2500.0 if args.form in ("bf16x3", "bf16x6", "f16x3") else None,
-                                     "hardware_flop_per_element": est.hardware_flop}},
+                     "matrix_core": {
+                         "peak_tflops": 2500.0 if args.form in ("bf16x3", "bf16x6", "f16x3") else None,
+                         # counter-measured (rocprofv3 MFMA_MOPS x 512 per launch) over this run's launch time
+                         "achieved_tflops": (mfma["flop"] / (per_kernel[dom] / 1e3) / 1e12) if mfma else None,
+                         "busy_frac": mfma["busy_frac"] if mfma else None,
+                         "counter_source": mfma["source"] if mfma else None,
+                         # the dense contractions x split products (sdg_performance_estimate)
+                         "estimate_tflops": mma_tflops, "estimate_flop_per_element": est.hardware_flop}},
     }
     if not args.no_cpu:
         log("cpu baseline ...")
-        result["cpu_baseline"] = cpu_baseline(args.order, args.cpu_elements, args.cpu_steps, args.seed)
+        result["cpu_baseline"] = cpu_baseline(args.order, args.cpu_elements, args.cpu_steps, args.seed, args.mesh)
         log(f"cpu baseline {result['cpu_baseline']['value']:.0f} element-updates/s")
     print(json.dumps(result), flush=True)
     return 0

@@ -259,8 +259,9 @@ int sdg_proxy_run_static(sdg_engine* engine, int kernel_id, int steps, double dt
  * src/Monitoring/Metric.h:16-50, summed by src/Proxy/Runner.cpp:120-150),
  * per element: non-zero and hardware FLOP, bytes (DOF blocks and
  * per-element operators moved once) and kernel bytes (what the kernels of
- * this arithmetic move, face traces included).  The DR kernels 5-7 (FP64 and
- * FP32) count the proxy's layout of 4 DR faces per cell, all faces DR
+ * this arithmetic move, face traces included).  The DR kernels 5-7 (every
+ * arithmetic; the DR parts in the engine's real type) count the proxy's
+ * layout of 4 DR faces per cell, all faces DR
  * (DynamicRupture::metrics, src/Kernels/DynamicRupture.cpp:170-195, and the
  * nodalFlux estimate of Neighbor.cpp:198-214). */
 typedef struct {
@@ -410,7 +411,8 @@ int sdg_dr_face_data(int64_t num_faces, const double* vertices, const double* ma
  * engine's real type) at the `order` time points (coeffs [order][order]:
  * c[o][d] = t_o^d / d!) and the Gauss points, rotated to the face frame;
  * q_plus, q_minus [f][order][9][points] FP64 (the friction law's
- * q_interpolated_plus/minus).  FP64 and FP32 engines. */
+ * q_interpolated_plus/minus).  Every engine; the matrix-core engines
+ * interpolate their FP32 derivatives in FP32. */
 int sdg_dr_space_time_interpolation(sdg_engine* engine, const double* coeffs, const void* derivatives,
                                     const sdg_dr_layer* layer, double* q_plus, double* q_minus, void* stream);
 /* Nodal flux of the dynamic-rupture faces (Neighbor.cpp:93-101; spec
@@ -440,7 +442,8 @@ int sdg_proxy_dr_faces(int64_t n, uint64_t seed, int32_t* dr_mapping, int32_t* p
  * neigh_dr = neighbour integral + nodal flux of the DR faces; godunov_dr =
  * space-time interpolation at the Gauss-Legendre points of [0, dt] from
  * layer->derivatives; all_dr = ader + local integral (writing the
- * derivatives), godunov_dr, neigh_dr.  FP64 and FP32 engines for 5-7. */
+ * derivatives), godunov_dr, neigh_dr.  Every engine (a matrix-core engine
+ * runs ader + local on the matrix cores and the DR kernels in FP32). */
 int sdg_proxy_run_dr(sdg_engine* engine, int kernel_id, int steps, double dt, const sdg_layer* layer,
                      const sdg_proxy_dr* dr, void* stream);
 /* Face evaluation matrices of dynamic rupture (V3mTo2n(i, h) and

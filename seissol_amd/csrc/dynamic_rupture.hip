@@ -424,10 +424,6 @@ extern "C" int sdg_dr_space_time_interpolation(sdg_engine* engine, const double*
     setError("sdg_dr_space_time_interpolation: derivatives, outputs and TinvT are required");
     return SDG_ERR_INVALID;
   }
-  if (st.arithmetic != SDG_FP64 && st.arithmetic != SDG_FP32) {
-    setError("sdg_dr_space_time_interpolation: FP64 and FP32 engines");
-    return SDG_ERR_UNSUPPORTED;
-  }
   if (const int rc = ensureDrMatrices(st); rc != SDG_OK) return rc;
   const auto s = static_cast<hipStream_t>(stream);
   const bool dbl = st.arithmetic == SDG_FP64;
@@ -458,10 +454,6 @@ extern "C" int sdg_dr_nodal_flux(sdg_engine* engine, const sdg_layer* layer, con
        dr_layer->flux_solver_plus == nullptr || dr_layer->flux_solver_minus == nullptr)) {
     setError("sdg_dr_nodal_flux: dofs, dr_mapping, imposed states and flux solvers are required");
     return SDG_ERR_INVALID;
-  }
-  if (st.arithmetic != SDG_FP64 && st.arithmetic != SDG_FP32) {
-    setError("sdg_dr_nodal_flux: FP64 and FP32 engines");
-    return SDG_ERR_UNSUPPORTED;
   }
   if (const int rc = ensureDrMatrices(st); rc != SDG_OK) return rc;
   const auto s = static_cast<hipStream_t>(stream);

@@ -117,7 +117,7 @@ using namespace sdg;
 extern "C" int sdg_performance_estimate(int order, int arithmetic, int kernel_id, sdg_work_estimate* out) {
   if (out == nullptr || order < kMinOrder || order > kMaxOrder || arithmetic < SDG_FP64 ||
       arithmetic > SDG_F16X3 || kernel_id < 0 || kernel_id > 7 ||
-      (kernel_id >= 5 && (order > 7 || (arithmetic != SDG_FP64 && arithmetic != SDG_FP32)))) {
+      (kernel_id >= 5 && (order > 7 || (matrixCore(arithmetic) && order < 3)))) {
     setError("sdg_performance_estimate: invalid order, arithmetic, kernel or NULL output");
     return SDG_ERR_INVALID;
   }
@@ -147,10 +147,19 @@ extern "C" int sdg_performance_estimate(int order, int arithmetic, int kernel_id
     const double dsize = static_cast<double>(sdg_derivatives_size(order));
     const double godunovBytes = 4 * (2 * dsize * s + 81 * 8 + 2.0 * order * 9 * np * 8);
     const double nodalBytes = 2 * q * s + 4 * (9.0 * np * 8 + 81 * 8);
+    double localHardware = 0.0, localKernelBytes = 0.0;
     if (kernel_id == 7) {
       add(w.ader);
       add(w.volume);
       bytes += (3 * q + 3 * kStarNonZeros + 4 * 81 + dsize) * s;  // Q in/out, I, derivatives out
+      if (traces) {
+        // the matrix-core local pass runs its whole product chain (the DR
+        // faces' flux operands are zero) and writes traces, not I
+        sdg_work_estimate l{};
+        if (const int rc = sdg_performance_estimate(order, arithmetic, 1, &l); rc != SDG_OK) return rc;
+        localHardware = l.hardware_flop - (w.ader.dense + w.volume.dense);
+        localKernelBytes = traceBytes - q * s;
+      }
     }
     if (kernel_id == 6 || kernel_id == 7) {
       c.nz += 4 * d.godunovFace.nz;
@@ -163,9 +172,9 @@ extern "C" int sdg_performance_estimate(int order, int arithmetic, int kernel_id
       bytes += nodalBytes;
     }
     out->nonzero_flop = c.nz;
-    out->hardware_flop = c.dense;
+    out->hardware_flop = c.dense + localHardware;
     out->bytes = bytes;
-    out->kernel_bytes = bytes;
+    out->kernel_bytes = bytes + localKernelBytes;
     return SDG_OK;
   }
   const bool ader = kernel_id == 0 || kernel_id == 1 || kernel_id == 3;

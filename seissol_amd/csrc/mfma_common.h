@@ -1,7 +1,11 @@
 // Tile geometry shared by the matrix-core kernels and the host code that
 // prepares their pre-split, pre-swizzled matrix fragments.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
+
+#include "engine.h"
 
 namespace sdg {
 namespace mfma {
@@ -49,6 +53,27 @@ struct MShape {
   static constexpr int tFPCol = tFP + 3 * B2 * O; // their columns (exact small integers): [3][B2][O]
   static constexpr int tAUn = tFPCol + 3 * B2 * O;
   static constexpr int tTotal = tAUn + 9;
+};
+
+// local-time-stepping outputs of the local pass (TimeCluster.cpp:380-434):
+// per cell its LtsSetup bits say whether it stores its buffer (accumulated
+// over the sub-steps a slower neighbour sees unless `reset`) and its
+// derivatives; setup == nullptr: buffer (if any) stored, derivatives (if any)
+// stored for every cell
+struct LtsOut {
+  const uint16_t* setup = nullptr;
+  float* deriv = nullptr;
+  int reset = 1;
+  int pad_ = 0;
+};
+
+// host entry points of one order's kernels
+template <int O>
+struct MfmaOrder {
+  // local pass; lts == nullptr: the GTS pass (face traces, no derivatives)
+  static int local(const EngineState& e, const double* coeffs, const sdg_layer* layer, bool writeIntegrated,
+                   hipStream_t stream, const LtsOut* lts);
+  static int neighbor(const EngineState& e, const sdg_layer* layer, hipStream_t stream);
 };
 
 }  // namespace mfma

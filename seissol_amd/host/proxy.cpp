@@ -9,7 +9,8 @@
 // (src/Proxy/Common.cpp:22-41); a comma-separated list runs its kernels in
 // sequence in every pass.  With a DR kernel every cell face is a DR face and
 // the layer gets the proxy's random DR faces, derivatives and imposed states
-// (src/Proxy/Allocator.cpp:186-294; FP64 and FP32).
+// (src/Proxy/Allocator.cpp:186-294; every arithmetic, the DR kernels in the
+// engine's real type).
 //
 // Data as src/Proxy/Allocator.cpp:49-134 (random affine tetrahedra with
 // random neighbours and face relations; or a conforming periodic box),
@@ -181,8 +182,6 @@ int run(int argc, char** argv) {
   }
   const Arithmetic arithmetic = seissol_amd::arithmeticFromString(arith);
   const bool withDR = std::any_of(kernelIds.begin(), kernelIds.end(), [](int id) { return id >= 5; });
-  if (withDR && arithmetic != Arithmetic::FP64 && arithmetic != Arithmetic::FP32)
-    return usage("the DR kernels run in fp64 or fp32");
 
   // synthetic data (host, FP64)
   std::cerr << "Allocating fake data..." << std::endl;

@@ -110,8 +110,14 @@ def test_dr_work_estimates_known_answers():
     assert abs(a.nonzero_flop - (ader.nonzero_flop + volume_nz + g.nonzero_flop + nb.nonzero_flop)) < 1e-6
     # FP32 halves the cell data, not the FP64 face data
     assert _estimate(order, _capi.SDG_FP32, 5).bytes == 2 * 9 * B * 4 + 4 * (9 * NP * 8 + 81 * 8)
-    # the DR kernels run in FP64 and FP32 only
-    assert _capi.lib().sdg_performance_estimate(6, _capi.SDG_F16X3, 7, ctypes.byref(_capi.WorkEstimate())) != 0
+    # a matrix-core engine: the DR parts in FP32 (one product each), the local
+    # pass with its whole split-product chain; the same non-zero work
+    m7 = _estimate(order, _capi.SDG_F16X3, 7)
+    assert m7.nonzero_flop == _estimate(order, _capi.SDG_FP32, 7).nonzero_flop
+    assert _estimate(order, _capi.SDG_F16X3, 6).hardware_flop == g.hardware_flop
+    assert abs(m7.hardware_flop - (_estimate(order, _capi.SDG_F16X3, 1).hardware_flop + g.hardware_flop +
+                                   nb.hardware_flop)) < 1e-6
+    assert _capi.lib().sdg_performance_estimate(2, _capi.SDG_F16X3, 7, ctypes.byref(_capi.WorkEstimate())) != 0
 
 
 def test_proxy_kernel_names_and_dr_faces():
