@@ -171,3 +171,35 @@ def test_warm_start_keeps_the_root_and_cuts_newton_steps(gpu):
             e = _rel(d.fault.host(name), drv.fault.host(name))
             assert e < 1e-6, (name, e)
         assert d.fault.host("convergence_inner").all()
+
+
+def test_friction_matches_oracle_at_full_fault_size(gpu):
+    # the learned-form fault: 20,000 faces x 25 points at the CFL step of its
+    # corner tetrahedron, plain and with the closed-form start
+    import torch
+    steps, dt = 40, 5.0313968e-4
+    setup = make_fault_setup(num_faces=20_000, order=4, seed=1234, length=10_000.0)
+    np_ = setup.num_points
+    for closed_form in (False, True):
+        print(f"friction 20000 faces closed_form_start={closed_form}", flush=True)
+        solver = FrictionSolver(setup.params, 4)
+        solver.closed_form_start = closed_form
+        drv = FaultDriver(setup, solver, dt)
+        drv.advance(steps)
+        torch.cuda.synchronize()
+        ref = _run_oracle(setup, steps, dt, closed_form_start=closed_form)
+        assert ref["slip_rate_magnitude"].max() > 0.01, "the nucleation patch must slide"
+        for name in COMPARED:
+            err = _rel(drv.fault.host(name)[..., :np_], ref[name][..., :np_])
+            assert err < 1e-9, (name, err)
+        for name in ("convergence_inner", "convergence_outer"):
+            np.testing.assert_array_equal(drv.fault.host(name)[..., :np_], ref[name][..., :np_])
+        # the device's exp/log/expm1 and the host's may differ in the last bit:
+        # over 20 million point-solves a borderline residual test can then take
+        # one evaluation more or less on a face (seen: one face of 20,000), so
+        # the counts are pinned per face on all but 0.05 % of the faces and in total
+        for name in ("newton_iterations", "slip_rate_updates"):
+            got, want = drv.fault.host(name)[..., :np_], ref[name][..., :np_]
+            faces_off = int((got != want).any(axis=1).sum())
+            assert faces_off <= 10, (name, faces_off)
+            assert abs(int(got.sum()) - int(want.sum())) <= 1e-5 * int(want.sum()), name

@@ -276,3 +276,24 @@ def test_order8_runs_in_fp64_and_fp32_only(gpu):
             Engine(8, arith)
     Engine(8, "fp64")
     Engine(8, "fp32")
+
+
+def test_f16x3_stays_within_1e5_of_the_oracle_over_200_steps(gpu):
+    # the chosen form's accuracy bar over the step count of the precision
+    # form (200 steps, ORDER 6), on a conforming periodic box of 5120 cells
+    order, steps = 6, 200
+    setup = make_setup(order, mesh="periodic", dims=(8, 8, 16), seed=21)
+    print(f"f16x3 order {order} n={setup.n} {steps} steps", flush=True)
+    eng = Engine(order, "f16x3")
+    eng.set_layout(0, "compact")
+    layer = DeviceLayer(eng, setup)
+    run_proxy(eng, layer, "all", steps, setup.dt)
+    _sync()
+    gm = global_matrices(order)["flat"]
+    coeffs = monomial_integrate(order, 0.0, setup.dt)
+    q_ref, _ = oracle.proxy_all(order, gm, steps, coeffs, _np(layer.star64), _np(layer.aplus64),
+                                _np(layer.aminus64), setup.neighbors, setup.face_relations, setup.dofs)
+    q = _np(layer.dofs).astype(np.float64)
+    per_q = [rel_l2(q[:, p], q_ref[:, p]) for p in range(9)]
+    print(f"f16x3 after {steps} steps: rel L2 per quantity max {max(per_q):.2e}")
+    assert max(per_q) < 1e-5, per_q
