@@ -102,6 +102,14 @@ int sdg_engine_create(sdg_engine** engine, int order, int arithmetic);
 /* Replace the generated global matrices by caller-provided ones (layout as
  * sdg_global_matrices). */
 int sdg_engine_set_global_matrices(sdg_engine* engine, const double* host_matrices);
+/* Kernel family of an engine (no reference counterpart: the reference picks
+ * its generated kernels at build time).  SDG_KERNELS_AUTO (default): the
+ * matrix-core passes where the engine has them (the split-16-bit forms; the
+ * FP64 engine's fused ader + local pass at orders 3-7 on v_mfma_f64_16x16x4);
+ * SDG_KERNELS_VALU: the FP64/FP32 VALU kernels only (FP64 and FP32 engines). */
+#define SDG_KERNELS_AUTO 0
+#define SDG_KERNELS_VALU 1
+int sdg_engine_set_kernels(sdg_engine* engine, int family);
 int sdg_engine_destroy(sdg_engine* engine);
 int64_t sdg_derivatives_size(int order); /* reals per element, packed: sum_i 9*B(O-i) */
 
@@ -146,7 +154,10 @@ typedef struct {
   const int32_t* face_neighbors;  /* [n][4], index into time_integrated       */
   const uint8_t* face_relations;  /* [n][4][2]                                */
   const uint8_t* face_types;      /* [n][4] or NULL (all regular)             */
-  void* face_traces;              /* [n][4][9][B2P] FP32, B2P = B2 rounded up */
+  void* face_traces;              /* [n][4][9][B2P], B2P = B2 rounded up to a multiple of 4: FP32 for the
+                                     matrix-core forms; FP64 for the FP64 engine (orders 3-7), whose local
+                                     pass then also writes them and whose proxy kernels all/neigh read
+                                     them (its batched neighbour integral always reads time_integrated) */
                                   /* to a multiple of 4 (padding rows zero):  */
                                   /* rT_f I of every face;                    */
                                   /* required by the matrix-core arithmetics  */

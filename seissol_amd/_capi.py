@@ -20,6 +20,8 @@ SDG_F16X3 = 4
 SDG_LTS_HAS_BUFFERS = 1 << 8
 SDG_LTS_HAS_DERIVATIVES = 1 << 9
 SDG_LTS_ACCUMULATE_BUFFERS = 1 << 10
+SDG_KERNELS_AUTO = 0
+SDG_KERNELS_VALU = 1
 SDG_STAR_DENSE = 0
 SDG_STAR_COMPACT = 1
 SDG_FACE_REGULAR = 0
@@ -138,6 +140,7 @@ SIGNATURES = {
     "sdg_compute_batched_neighbors_integral": (I32, [P, C.POINTER(Layer), P]),
     "sdg_compute_batched_ader_local": (I32, [P, P, D, C.POINTER(Layer), P]),
     "sdg_proxy_kernel_id": (I32, [C.c_char_p]),
+    "sdg_engine_set_kernels": (I32, [P, I32]),
     "sdg_proxy_run": (I32, [P, I32, I32, D, C.POINTER(Layer), P]),
     "sdg_proxy_run_static": (I32, [P, I32, I32, D, C.POINTER(Layer), P]),
     "sdg_proxy_dr_faces": (I32, [I64, C.c_uint64, P, P, P, P]),

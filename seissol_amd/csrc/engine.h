@@ -68,6 +68,11 @@ struct EngineState {
   void* mfmaTables = nullptr;
   // dynamic rupture face matrices (V3mTo2n, V3mTo2nTWDivM), on first use
   void* drMatrices = nullptr;
+  // FP64 engines of orders 3-7: FP64 fragments of the matrix-core local pass
+  void* mfma64Frags = nullptr;
+  // SDG_KERNELS_AUTO: the matrix-core passes where the engine has them;
+  // SDG_KERNELS_VALU: the VALU kernels only (sdg_engine_set_kernels)
+  int kernels = SDG_KERNELS_AUTO;
 };
 
 // the layout of `layer` (its strides, 0 = the dense defaults) under the engine's settings
@@ -104,6 +109,19 @@ int64_t mfmaFragmentCount(int order);
 int launchLocalMfma(const EngineState& e, const double* coeffs, const sdg_layer* layer, bool writeIntegrated,
                     hipStream_t stream);
 int launchNeighborMfma(const EngineState& e, const sdg_layer* layer, hipStream_t stream);
+// kernels_mfma64.hip: the FP64 engines' fused ader + local pass on the matrix
+// cores (time integral written when the layer has time_integrated, face
+// traces when it has face_traces (FP64 [n][4][9][B2P]); with
+// lts_setup the LTS buffers and derivatives, else derivatives of every cell
+// when the layer has them)
+bool hasMfma64(const EngineState& e);
+int launchAderLocalMfma64(const EngineState& e, const double* coeffs, const sdg_layer* layer, hipStream_t stream);
+// the FP64 neighbour pass on the matrix cores from the FP64 face traces the
+// local pass wrote (layer->face_traces)
+int launchNeighborMfma64(const EngineState& e, const sdg_layer* layer, hipStream_t stream);
+int launchLocalIntegrationMfma64(const EngineState& e, const double* coeffs, const sdg_layer* layer,
+                                 int resetBuffers, hipStream_t stream);
+int uploadMfma64(EngineState& s, const double* globalMatrices);
 // the LTS local pass on the matrix cores: buffers and derivatives per the
 // layer's lts_setup (TimeCluster::computeLocalIntegration), no face traces
 int launchLocalIntegrationMfma(const EngineState& e, const double* coeffs, const sdg_layer* layer, int resetBuffers,

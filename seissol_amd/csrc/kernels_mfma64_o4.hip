@@ -1,0 +1,4 @@
+// FP64 matrix-core local pass of ORDER 4.
+#include "mfma64_kernels.h"
+
+template struct sdg::mfma64::MfmaOrder64<4>;

@@ -906,6 +906,7 @@ int launchAder(const EngineState& e, const double* coeffs, const sdg_layer* laye
   return dispatch<AderOp>(e, coeffs, layer, stream, false, 1);
 }
 int launchAderLocal(const EngineState& e, const double* coeffs, const sdg_layer* layer, hipStream_t stream) {
+  if (hasMfma64(e)) return launchAderLocalMfma64(e, coeffs, layer, stream);
   return dispatch<AderOp>(e, coeffs, layer, stream, true, 1);
 }
 int launchLocalIntegration(const EngineState& e, const double* coeffs, const sdg_layer* layer, int resetBuffers,

@@ -57,11 +57,16 @@ bool ltsLayerOk(const sdg_layer* L, bool neighbors, const char* who) {
 // shapes; a matrix-core engine has its matrix-core pass (variant 0) and the
 // FP32 VALU shapes after it.  The neighbour pass is the VALU kernel of the
 // engine's real type (FP32 for the matrix-core engines) with kTileVariants shapes.
-int localVariants(const EngineState& e) { return matrixCore(e.arithmetic) ? 1 + kTileVariants : kTileVariants; }
+// The FP64 engine (orders 3-7, kernel family AUTO) likewise has its FP64
+// matrix-core pass as variant 0.
+bool matrixCoreLocal(const EngineState& e) { return matrixCore(e.arithmetic) || hasMfma64(e); }
+int localVariants(const EngineState& e) { return matrixCoreLocal(e) ? 1 + kTileVariants : kTileVariants; }
 
 int runLocal(const EngineState& e, const double* coeffs, const sdg_layer* own, int reset, hipStream_t s, int v) {
-  if (!matrixCore(e.arithmetic)) return launchLocalIntegration(e, coeffs, own, reset, s, v);
-  if (v == 0) return launchLocalIntegrationMfma(e, coeffs, own, reset, s);
+  if (!matrixCoreLocal(e)) return launchLocalIntegration(e, coeffs, own, reset, s, v);
+  if (v == 0)
+    return matrixCore(e.arithmetic) ? launchLocalIntegrationMfma(e, coeffs, own, reset, s)
+                                    : launchLocalIntegrationMfma64(e, coeffs, own, reset, s);
   return launchLocalIntegration(e, coeffs, own, reset, s, v - 1);
 }
 

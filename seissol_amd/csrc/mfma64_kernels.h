@@ -1,0 +1,563 @@
+// Plain FP64 element update on the matrix cores (v_mfma_f64_16x16x4_f64):
+// the fused ader + local pass of the FP64 drop-in (arithmetic SDG_FP64), FP64
+// throughout, one product per contraction.  Included by the per-order sources
+// kernels_mfma64_o<O>.hip.
+//
+// Every contraction with a global matrix A (kDivMT, kDivM, fMrT, rDivM) is a
+// GEMM  Y(M x 144) += A(M x K) X(K x 144)  over a tile of 16 elements, the 144
+// columns being (quantity p, element e) -> 16 p + e, in k-steps of 4.  One
+// wave owns one 16-row tile of the basis index for all 9 quantities of the
+// 16 elements: lane l holds rows g, g + 4, g + 8, g + 12 (g = l / 16) of the
+// tile for element e = l % 16 (the accumulator layout of v_mfma_f64_16x16x4),
+// so the 9x9 star and AplusT products that mix quantities are register
+// arithmetic; wave w also owns the w-th 16-row tile of the stacked face traces.
+// X lives in LDS column-major ([column][k], k contiguous); A is
+// pre-swizzled in global memory so one k-step's fragment is one coalesced
+// 8-byte load per lane (L2-resident, shared by all workgroups).
+//
+// Phases of a tile (Time.cpp:65-140, Local.cpp:124-160; spec
+// codegen/kernels/aderdg.py:321-537): CK derivatives and time integral I,
+// face traces (fMrT_f / M2) I, volume sum_j kDivM_j I star_j, W_f = traces_f
+// AplusT_f, local flux rDivM_f diag(M2) W_f face by face; Q += volume + flux.
+// The time integral (and, for LTS, the buffer and the derivatives) are
+// written for the neighbour pass.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "engine.h"
+#include "mfma_common.h"
+
+namespace sdg {
+namespace mfma64 {
+
+using mfma::basis3;
+using mfma::cdiv;
+using mfma::kCols;
+using mfma::kTileElems;
+
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+__device__ constexpr int kStarQ[9][3] = {{6, 7, 8}, {6, 7, 8}, {6, 7, 8}, {6, 7, -1}, {7, 8, -1},
+                                         {6, 8, -1}, {0, 3, 5}, {1, 3, 4}, {2, 4, 5}};
+__device__ constexpr int kStarOff[9] = {0, 3, 6, 9, 11, 13, 15, 18, 21};
+__device__ constexpr int kStarCnt[9] = {3, 3, 3, 2, 2, 2, 3, 3, 3};
+__device__ constexpr int kStarPos[24] = {6 + 0,  7 + 0,  8 + 0,  6 + 9,  7 + 9,  8 + 9,  6 + 18, 7 + 18,
+                                         8 + 18, 6 + 27, 7 + 27, 7 + 36, 8 + 36, 6 + 45, 8 + 45, 0 + 54,
+                                         3 + 54, 5 + 54, 1 + 63, 3 + 63, 4 + 63, 2 + 72, 4 + 72, 5 + 72};
+
+template <int O>
+struct Coeffs64 {
+  double c[O];
+};
+
+__device__ inline doublex4 mma(double a, double b, doublex4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// the lane's rows base + 4 r (r < 4, those below `rows`) of one column into X
+template <int LD>
+__device__ inline void storeX(double* xs, int col, int base, int rows, const double (&v)[4]) {
+  double* d = xs + col * LD + base;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    if (base + 4 * r < rows) d[4 * r] = v[r];
+}
+
+// acc[q] += A_tile X(:, quantity q) over `ksteps` k-steps, q < 9
+template <int LD>
+__device__ inline void gemm9(doublex4 (&acc)[9], const double* __restrict__ a, int ksteps, const double* xs,
+                             int lane) {
+  const int g = lane >> 4, e = lane & 15;
+  double an = a[lane];
+  for (int kk = 0; kk < ksteps; ++kk) {
+    const double ac = an;
+    if (kk + 1 < ksteps) an = a[(kk + 1) * 64 + lane];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = mma(ac, xs[(q * 16 + e) * LD + 4 * kk + g], acc[q]);
+  }
+}
+
+// y += sum_j (A_j X) star_j for the lane's element: three passes over the
+// input quantity groups {6,7,8}, {0,1,2}, {3,4,5} (the elastic star matrix maps
+// velocities to stresses and stresses to velocities), one matrix at a time
+// with 3 accumulator tiles
+template <int LD>
+__device__ inline void gemmStars(const double* const (&a)[3], int ks, const double* xs, int lane, const double* sS,
+                                 double (&y)[9][4]) {
+  const int g = lane >> 4, e = lane & 15;
+#pragma unroll
+  for (int pass = 0; pass < 3; ++pass) {
+    const int q0 = pass == 0 ? 6 : (pass == 1 ? 0 : 3);
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+      doublex4 acc[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) acc[i] = doublex4{0, 0, 0, 0};
+      const double* aj = a[jj];
+      double an = aj[lane];
+      for (int kk = 0; kk < ks; ++kk) {
+        const double ac = an;
+        if (kk + 1 < ks) an = aj[(kk + 1) * 64 + lane];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc[i] = mma(ac, xs[((q0 + i) * 16 + e) * LD + 4 * kk + g], acc[i]);
+      }
+      // the star coefficients are read where they are used (else the compiler
+      // keeps all 72 live across the recursion)
+      const double* sSp = sS;
+      asm volatile("" : "+v"(sSp));
+#pragma unroll
+      for (int p = 0; p < 9; ++p)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          if (k < kStarCnt[p]) {
+            const int qi = kStarQ[p][k];
+            if (qi >= q0 && qi < q0 + 3) {
+              const double sv = sSp[jj * 24 + kStarOff[p] + k];
+#pragma unroll
+              for (int j = 0; j < 4; ++j) y[p][j] += acc[qi - q0][j] * sv;
+            }
+          }
+      // finish this contraction before the next product starts: otherwise the
+      // compiler defers all nine contractions past the nine k-loops and keeps
+      // their 27 accumulator tiles live at once
+#pragma unroll
+      for (int p = 0; p < 9; ++p)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) asm volatile("" : "+v"(y[p][j]));
+    }
+  }
+}
+
+template <int O, bool LTS, bool DEF>
+__global__ __launch_bounds__(Shape64<O>::THREADS, 1) void kLocalMfma64(
+    int64_t n, const double* __restrict__ frag, Coeffs64<O> cf, double* __restrict__ dofs,
+    double* __restrict__ integ, double* __restrict__ traces, const double* __restrict__ star,
+    const double* __restrict__ aplus, const uint8_t* __restrict__ faceTypes, CellLayout lay, LtsOut64 lts) {
+  using S = Shape64<O>;
+  constexpr int B = S::B, MT = S::MT, LDX = S::LDX, LDW = S::LDW, B2P = S::B2P;
+  __shared__ __attribute__((aligned(16))) double xs[kCols * LDX];
+  __shared__ double sA[kTileElems * 324];   // AplusT of the four faces, [element][face][81]
+  __shared__ double sStar[kTileElems * 72];  // compact star matrices, [element][3][24]
+
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int g = lane >> 4, e = lane & 15;
+  const int64_t E0 = static_cast<int64_t>(blockIdx.x) * kTileElems;
+  const int64_t elem = E0 + e;
+  const bool valid = elem < n;
+  const int ldq = DEF ? B : lay.ldq;
+  const int64_t qStride = DEF ? 9 * B : lay.qStride;
+  const int row0 = 16 * w + g;  // the lane's rows row0 + 4 r
+  const bool ownsRows = w < MT;  // waves beyond MT own only a trace tile
+  double* const laneDofs = dofs + (valid ? elem : 0) * qStride + (ownsRows ? row0 : 0);
+
+  // the tile's compact star matrices and AplusT (clamped addresses, in bounds:
+  // a launched tile has E0 < n)
+  for (int i = threadIdx.x; i < kTileElems * 72; i += S::THREADS) {
+    const int s = i / 72, r = i % 72;
+    const int64_t el = E0 + s < n ? E0 + s : n - 1;
+    const int64_t starStride = DEF ? (lay.starCompact ? 72 : 243) : lay.starStride;
+    const double v = star[el * starStride + (lay.starCompact ? r : 81 * (r / 24) + kStarPos[r % 24])];
+    sStar[i] = E0 + s < n ? v : 0.0;
+  }
+  for (int i = threadIdx.x; i < kTileElems * 324; i += S::THREADS) {
+    const int s = i / 324, r = i % 324;
+    const int64_t el = E0 + s < n ? E0 + s : n - 1;
+    const double v = aplus[DEF ? el * 324 + r : el * lay.aplusStride + r];
+    sA[i] = E0 + s < n ? v : 0.0;
+  }
+
+  // Q: the lane's rows (zero beyond B and for padding elements); read again
+  // for the final update, so only the time integral stays live
+  double q[9][4], iacc[9][4];
+#pragma unroll
+  for (int p = 0; p < 9; ++p)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      q[p][j] = (valid && ownsRows && row0 + 4 * j < B) ? laneDofs[p * ldq + 4 * j] : 0.0;
+      iacc[p][j] = cf.c[0] * q[p][j];
+    }
+  [[maybe_unused]] unsigned setup = 0;
+  [[maybe_unused]] bool storeDeriv = false;
+  [[maybe_unused]] double* cellDeriv = nullptr;
+  if constexpr (LTS) {
+    setup = (lts.setup != nullptr && valid) ? lts.setup[elem] : 0u;
+    storeDeriv = lts.deriv != nullptr && valid && (lts.setup == nullptr || (setup & SDG_LTS_HAS_DERIVATIVES));
+    cellDeriv = lts.deriv + (valid ? elem : 0) * lay.dsize;
+    if (storeDeriv && ownsRows)
+#pragma unroll
+      for (int p = 0; p < 9; ++p)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (row0 + 4 * j < B) cellDeriv[p * ldq + row0 + 4 * j] = q[p][j];
+  }
+  [[maybe_unused]] int64_t doff = 9LL * ldq;
+  // X holds rows [0, KB) (rows beyond are never read)
+  if (ownsRows)
+#pragma unroll
+    for (int p = 0; p < 9; ++p) storeX<LDX>(xs, p * 16 + e, row0, S::KB, q[p]);
+  __syncthreads();
+  const double* sS = sStar + e * 72;
+
+  // Cauchy-Kowalevski recursion: wave w owns m-tile w of dQ_d
+#pragma unroll 1
+  for (int d = 1; d < O; ++d) {
+    const int m = basis3(O - d), nIn = basis3(O - d + 1);
+    const int mtiles = cdiv(m, 16), ks = cdiv(nIn, 4);
+    double dq[9][4];
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) dq[p][j] = 0.0;
+    if (w < mtiles) {
+      const double* a[3] = {frag + S::ck + (0LL * MT + w) * S::KSB * 64, frag + S::ck + (1LL * MT + w) * S::KSB * 64,
+                            frag + S::ck + (2LL * MT + w) * S::KSB * 64};
+      gemmStars<LDX>(a, ks, xs, lane, sS, dq);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (row0 + 4 * j >= m)
+#pragma unroll
+          for (int p = 0; p < 9; ++p) dq[p][j] = 0.0;
+    }
+    if constexpr (LTS) {
+      const int ldm = lay.rows(m);
+      if (storeDeriv && w < mtiles)
+#pragma unroll
+        for (int p = 0; p < 9; ++p)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (row0 + 4 * j < m) cellDeriv[doff + p * ldm + row0 + 4 * j] = dq[p][j];
+      doff += 9LL * ldm;
+    }
+    __syncthreads();  // every wave has read dQ_{d-1}
+    if (w < mtiles) {
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        storeX<LDX>(xs, p * 16 + e, row0, S::KB, dq[p]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) iacc[p][j] += cf.c[d] * dq[p][j];
+      }
+    }
+    __syncthreads();
+  }
+
+  // time integral: output (LTS: the buffer, stored or accumulated) and X
+  bool storeBuffer = integ != nullptr && valid;
+  [[maybe_unused]] bool accumulate = false;
+  if constexpr (LTS) {
+    storeBuffer = storeBuffer && (lts.setup == nullptr || (setup & SDG_LTS_HAS_BUFFERS));
+    accumulate = lts.setup != nullptr && !lts.reset && (setup & SDG_LTS_ACCUMULATE_BUFFERS);
+  }
+  if (storeBuffer && ownsRows) {
+    double* const laneInteg = integ + elem * qStride + row0;
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (row0 + 4 * j < B) {
+          double* b = laneInteg + p * ldq + 4 * j;
+          if constexpr (LTS) *b = accumulate ? *b + iacc[p][j] : iacc[p][j];
+          else *b = iacc[p][j];
+        }
+  }
+  if (ownsRows)
+#pragma unroll
+    for (int p = 0; p < 9; ++p) storeX<LDX>(xs, p * 16 + e, row0, S::KB, iacc[p]);
+  __syncthreads();
+
+  // face traces (fMrT_f / M2) I: trace tile w, rows f * B2P + m
+  // the lane's trace rows r0 + 4 r: face r / B2P of the stacked operand
+  const int r0 = 16 * w + g;
+  const bool ownsTrace = w < S::MTF;
+  double tr[9][4];
+  if (w < S::MTF) {
+    doublex4 acc[9];
+#pragma unroll
+    for (int p = 0; p < 9; ++p) acc[p] = doublex4{0, 0, 0, 0};
+    gemm9<LDX>(acc, frag + S::fmrt + static_cast<int64_t>(w) * S::KSB * 64, S::KSB, xs, lane);
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) tr[p][j] = acc[p][j];
+    // the traces for the neighbour pass: [element][face][quantity][B2P]
+    if (traces != nullptr && valid)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = r0 + 4 * j;
+        if (r < S::FR) {
+          double* t = traces + ((elem * 4 + r / B2P) * 9) * B2P + r % B2P;
+#pragma unroll
+          for (int p = 0; p < 9; ++p) t[p * B2P] = tr[p][j];
+        }
+      }
+  }
+  // volume: sum_j kDivM_j I star_j (the initial value of the flux accumulators)
+  doublex4 acc[9];
+  {
+    double vol[9][4];
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) vol[p][j] = 0.0;
+    if (ownsRows) {
+      const double* a[3] = {frag + S::vol + (0LL * MT + w) * S::KSB * 64,
+                            frag + S::vol + (1LL * MT + w) * S::KSB * 64,
+                            frag + S::vol + (2LL * MT + w) * S::KSB * 64};
+      gemmStars<LDX>(a, S::KSB, xs, lane, sS, vol);
+    }
+#pragma unroll
+    for (int p = 0; p < 9; ++p) acc[p] = doublex4{vol[p][0], vol[p][1], vol[p][2], vol[p][3]};
+  }
+  // W = traces AplusT_f, each row with its own face's matrix; DR faces give none
+  if (ownsTrace) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int f = min((r0 + 4 * j) / B2P, 3);  // rows beyond FR: zero fragment rows, any face
+      const bool dr = valid && faceTypes != nullptr && faceTypes[elem * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE;
+      double wt[9];
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        // one column of AplusT_f at a time (else all 81 entries are loaded at once)
+        const double* A = sA + e * 324 + f * 81 + 9 * p;
+        asm volatile("" : "+v"(A));
+        double s = 0.0;
+#pragma unroll
+        for (int qq = 0; qq < 9; ++qq) s += tr[qq][j] * A[qq];
+        wt[p] = dr ? 0.0 : s;
+        asm volatile("" : "+v"(wt[p]));
+      }
+#pragma unroll
+      for (int p = 0; p < 9; ++p) tr[p][j] = wt[p];
+    }
+  }
+
+  // local flux face by face: acc += rDivM_f diag(M2) W_f (X area reused for W_f)
+#pragma unroll 1
+  for (int f = 0; f < 4; ++f) {
+    __syncthreads();  // the X area is free (f = 0: I no longer read; else: W_{f-1} read)
+    if (ownsTrace)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = r0 + 4 * j;
+        if (r >= f * B2P && r < (f + 1) * B2P)
+#pragma unroll
+          for (int p = 0; p < 9; ++p) xs[(p * 16 + e) * LDW + r - f * B2P] = tr[p][j];
+      }
+    __syncthreads();
+    if (ownsRows)
+      gemm9<LDW>(acc, frag + S::rdm + (static_cast<int64_t>(w) * 4 + f) * S::KSF * 64, S::KSF, xs, lane);
+  }
+  if (valid && ownsRows)
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (row0 + 4 * j < B) laneDofs[p * ldq + 4 * j] += acc[p][j];
+}
+
+// Neighbour flux from the neighbours' face traces (Neighbor.cpp:66-109; spec
+// aderdg.py:414-431): face by face, the 16 neighbours' traces rT_j I of the
+// shared side are gathered, W_f = (fP_h Tr) AminusT_f is formed by 16-lane
+// groups (one element, two rows per lane), and Q += rDivM_f W_f on the
+// matrix cores; the next face's traces and AminusT are loaded into registers
+// while the current face is computed.
+template <int O, bool DEF>
+__global__ __launch_bounds__(Shape64<O>::NTHREADS, 2) void kNeighborMfma64(
+    int64_t n, const double* __restrict__ frag, double* __restrict__ dofs, const double* __restrict__ traces,
+    const int32_t* __restrict__ neighbors, const uint8_t* __restrict__ faceRel, const double* __restrict__ aminus,
+    const uint8_t* __restrict__ faceTypes, CellLayout lay) {
+  using S = Shape64<O>;
+  constexpr int B = S::B, B2 = S::B2, B2P = S::B2P, LDW = S::LDW, THREADS = S::NTHREADS;
+  constexpr int HALF = B2P / 2;                 // row pairs of one (padded) face
+  static_assert(HALF <= 16, "a face's row pairs fit a 16-lane group");
+  constexpr int GROUPS = THREADS / 16;          // 16-lane groups per pass, one element each
+  constexpr int PASSES = cdiv(kTileElems, GROUPS);
+  constexpr int TR = 9 * B2P;                   // doubles of one face trace
+  constexpr int TR2 = TR / 2;                   // 16-byte pieces of one face trace
+  constexpr int GPT = cdiv(kTileElems * TR2, THREADS);
+  constexpr int APT = cdiv(kTileElems * 81, THREADS);
+  __shared__ __attribute__((aligned(16))) double xs[kCols * LDW];  // face operand, rows [0, B2P)
+  __shared__ __attribute__((aligned(16))) double sTr[kTileElems * TR];
+  __shared__ double sAm[kTileElems * 81];
+  __shared__ double sFP[3 * B2 * O];
+  __shared__ int sFPCol[3 * B2 * O];
+  __shared__ int sLink[kTileElems * 4];  // trace index nb * 4 + side of (element, face), -1: none
+  __shared__ int sRel[kTileElems * 4];   // orientation h of (element, face)
+
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int g = lane >> 4, e = lane & 15;
+  const int64_t E0 = static_cast<int64_t>(blockIdx.x) * kTileElems;
+  const bool valid = E0 + e < n;
+  const int ldq = DEF ? B : lay.ldq;
+  const int64_t qStride = DEF ? 9 * B : lay.qStride;
+  const int row0 = 16 * w + g;  // the lane's rows row0 + 4 r
+
+  for (int i = threadIdx.x; i < 3 * B2 * O; i += THREADS) {
+    sFP[i] = frag[S::tFP + i];
+    sFPCol[i] = static_cast<int>(frag[S::tFPCol + i]);
+  }
+  if (threadIdx.x < kTileElems * 4) {
+    const int s = threadIdx.x >> 2, f = threadIdx.x & 3;
+    const int64_t el = E0 + s;
+    int link = -1, h = 0;
+    if (el < n && (faceTypes == nullptr || faceTypes[el * 4 + f] == SDG_FACE_REGULAR)) {
+      link = neighbors[el * 4 + f] * 4 + min(static_cast<int>(faceRel[(el * 4 + f) * 2 + 0]), 3);
+      h = min(static_cast<int>(faceRel[(el * 4 + f) * 2 + 1]), 2);
+    }
+    sLink[threadIdx.x] = link;
+    sRel[threadIdx.x] = h;
+  }
+  __syncthreads();
+  double2 gv[GPT];
+  double av[APT];
+  auto gather = [&](int f) {
+#pragma unroll
+    for (int i = 0; i < GPT; ++i) {
+      const int idx = threadIdx.x + i * THREADS;
+      double2 v{0.0, 0.0};
+      if (idx < kTileElems * TR2) {
+        const int link = sLink[(idx / TR2) * 4 + f];
+        if (link >= 0) v = *reinterpret_cast<const double2*>(traces + static_cast<int64_t>(link) * TR + 2 * (idx % TR2));
+      }
+      gv[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int idx = threadIdx.x + i * THREADS;
+      const int64_t at = (E0 + idx / 81) * (DEF ? 324 : lay.aminusStride) + f * 81 + idx % 81;
+      av[i] = (idx < kTileElems * 81 && E0 + idx / 81 < n) ? aminus[at] : 0.0;
+    }
+  };
+  gather(0);
+  doublex4 acc[9];
+#pragma unroll
+  for (int p = 0; p < 9; ++p) acc[p] = doublex4{0, 0, 0, 0};
+#pragma unroll 1
+  for (int f = 0; f < 4; ++f) {
+    // the previous face's items have read sTr / sAm and its product xs
+#pragma unroll
+    for (int i = 0; i < GPT; ++i) {
+      const int idx = threadIdx.x + i * THREADS;
+      if (idx < kTileElems * TR2) *reinterpret_cast<double2*>(sTr + 2 * idx) = gv[i];
+    }
+#pragma unroll
+    for (int i = 0; i < APT; ++i) {
+      const int idx = threadIdx.x + i * THREADS;
+      if (idx < kTileElems * 81) sAm[idx] = av[i];
+    }
+    if (f < 3) gather(f + 1);
+    __syncthreads();
+#pragma unroll
+    for (int ps = 0; ps < PASSES; ++ps) {
+      // rows 2 rp and 2 rp + 1 of face f of element s (rows >= B2 are padding: zero)
+      const int s = static_cast<int>(threadIdx.x >> 4) + ps * GROUPS;
+      const int rp = static_cast<int>(threadIdx.x & 15);
+      const int r0 = 2 * rp;
+      const bool act = s < kTileElems && rp < HALF;
+      const int sl = s < kTileElems ? s : 0;
+      const bool regular = act && sLink[sl * 4 + f] >= 0;
+      const int h = sRel[sl * 4 + f];
+      const double* tr = sTr + sl * TR;
+      double u[2][9];
+#pragma unroll
+      for (int ri = 0; ri < 2; ++ri) {
+#pragma unroll
+        for (int qq = 0; qq < 9; ++qq) u[ri][qq] = 0.0;
+        const int row = r0 + ri;
+        if (regular && row < B2) {
+          const double* fp = sFP + (h * B2 + row) * O;
+          const int* fc = sFPCol + (h * B2 + row) * O;
+#pragma unroll
+          for (int kk = 0; kk < O; ++kk) {
+            const double c = fp[kk];
+            const int mm = fc[kk];
+#pragma unroll
+            for (int qq = 0; qq < 9; ++qq) u[ri][qq] += c * tr[qq * B2P + mm];
+          }
+        }
+      }
+      const double* A = sAm + sl * 81;
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        double w0 = 0.0, w1 = 0.0;
+        if (regular)
+#pragma unroll
+          for (int qq = 0; qq < 9; ++qq) {
+            const double a = A[qq + 9 * p];
+            w0 += u[0][qq] * a;
+            w1 += u[1][qq] * a;
+          }
+        if (act) *reinterpret_cast<double2*>(xs + (p * 16 + s) * LDW + r0) = double2{w0, w1};
+      }
+    }
+    __syncthreads();
+    gemm9<LDW>(acc, frag + S::rdmN + (static_cast<int64_t>(w) * 4 + f) * S::KSF * 64, S::KSF, xs, lane);
+  }
+  if (valid) {
+    double* const laneDofs = dofs + (E0 + e) * qStride + row0;
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (row0 + 4 * j < B) laneDofs[p * ldq + 4 * j] += acc[p][j];
+  }
+}
+
+template <int B>
+bool defaultLayout64(const CellLayout& lay) {
+  return lay.ldq == B && lay.qStride == 9 * B && lay.aplusStride == 324 && lay.starStride == (lay.starCompact ? 72 : 243);
+}
+
+template <int O>
+int MfmaOrder64<O>::local(const EngineState& st, const double* coeffs, const sdg_layer* L, hipStream_t stream,
+                          const LtsOut64* lts) {
+  using S = Shape64<O>;
+  const int64_t n = L->num_elements;
+  if (n == 0) return SDG_OK;
+  Coeffs64<O> cf{};
+  for (int i = 0; i < O; ++i) cf.c[i] = coeffs[i];
+  const CellLayout lay = cellLayout(st, L);
+  const dim3 grid(static_cast<unsigned>((n + kTileElems - 1) / kTileElems));
+  const bool def = defaultLayout64<S::B>(lay);
+  auto launch = [&](auto ltsFlag, auto defFlag) {
+    kLocalMfma64<O, decltype(ltsFlag)::value, decltype(defFlag)::value><<<grid, S::THREADS, 0, stream>>>(
+        n, static_cast<const double*>(st.mfma64Frags), cf, static_cast<double*>(L->dofs),
+        static_cast<double*>(L->time_integrated), static_cast<double*>(L->face_traces),
+        static_cast<const double*>(L->star), static_cast<const double*>(L->aplus), L->face_types, lay,
+        lts != nullptr ? *lts : LtsOut64{});
+  };
+  if (lts != nullptr) {
+    if (def) launch(std::true_type{}, std::true_type{});
+    else launch(std::true_type{}, std::false_type{});
+  } else {
+    if (def) launch(std::false_type{}, std::true_type{});
+    else launch(std::false_type{}, std::false_type{});
+  }
+  SDG_HIP_CHECK(hipGetLastError());
+  return SDG_OK;
+}
+
+template <int O>
+int MfmaOrder64<O>::neighbor(const EngineState& st, const sdg_layer* L, hipStream_t stream) {
+  using S = Shape64<O>;
+  const int64_t n = L->num_elements;
+  if (n == 0) return SDG_OK;
+  const CellLayout lay = cellLayout(st, L);
+  const dim3 grid(static_cast<unsigned>((n + kTileElems - 1) / kTileElems));
+  auto launch = [&](auto defFlag) {
+    kNeighborMfma64<O, decltype(defFlag)::value><<<grid, S::NTHREADS, 0, stream>>>(
+        n, static_cast<const double*>(st.mfma64Frags), static_cast<double*>(L->dofs),
+        static_cast<const double*>(L->face_traces), L->face_neighbors, L->face_relations,
+        static_cast<const double*>(L->aminus), L->face_types, lay);
+  };
+  if (defaultLayout64<S::B>(lay) && lay.aminusStride == 324) launch(std::true_type{});
+  else launch(std::false_type{});
+  SDG_HIP_CHECK(hipGetLastError());
+  return SDG_OK;
+}
+
+}  // namespace mfma64
+}  // namespace sdg

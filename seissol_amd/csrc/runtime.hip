@@ -339,8 +339,18 @@ int proxyRun(const EngineState& st, int kernel_id, int steps, double dt, const s
     setError("sdg_proxy_run: this kernel needs layer->time_integrated");
     return SDG_ERR_INVALID;
   }
+  // the FP64 matrix-core passes hand FP64 face traces from the local pass to
+  // the neighbour pass (the local pass still writes I); as for the
+  // matrix-core forms, `neigh` alone reads the traces of the last local pass
+  const bool mc64 = hasMfma64(st) && layer->face_traces != nullptr;
   for (int step = 0; step < steps; ++step) {
     int rc = SDG_OK;
+    if (mc64 && kernel_id <= 2) {
+      if (kernel_id == 0 || kernel_id == 1) rc = launchAderLocal(st, coeffs, layer, s);
+      if (rc == SDG_OK && (kernel_id == 0 || kernel_id == 2)) rc = launchNeighborMfma64(st, layer, s);
+      if (rc != SDG_OK) return rc;
+      continue;
+    }
     if (mc && kernel_id <= 2) {
       if (kernel_id == 0 || kernel_id == 1) rc = launchLocalMfma(st, coeffs, layer, false, s);
       if (rc == SDG_OK && (kernel_id == 0 || kernel_id == 2)) rc = launchNeighborMfma(st, layer, s);
@@ -505,10 +515,12 @@ int sdg_engine_create(sdg_engine** engine, int order, int arithmetic) {
   generateGlobalMatrices(order, gm.data());
   int rc = uploadMatrices(e->s, gm.data());
   if (rc == SDG_OK && matrixCore(arithmetic)) rc = uploadMfma(e->s, gm.data());
+  if (rc == SDG_OK && arithmetic == SDG_FP64) rc = uploadMfma64(e->s, gm.data());
   if (rc != SDG_OK) {
     if (e->s.matrices) (void)hipFree(e->s.matrices);
     if (e->s.mfmaFrags) (void)hipFree(e->s.mfmaFrags);
     if (e->s.mfmaTables) (void)hipFree(e->s.mfmaTables);
+    if (e->s.mfma64Frags) (void)hipFree(e->s.mfma64Frags);
     delete e;
     return rc;
   }
@@ -519,8 +531,23 @@ int sdg_engine_create(sdg_engine** engine, int order, int arithmetic) {
 int sdg_engine_set_global_matrices(sdg_engine* engine, const double* host_matrices) {
   if (engine == nullptr || host_matrices == nullptr) { setError("invalid arguments"); return SDG_ERR_INVALID; }
   const int rc = uploadMatrices(engine->s, host_matrices);
-  if (rc != SDG_OK || !matrixCore(engine->s.arithmetic)) return rc;
+  if (rc != SDG_OK) return rc;
+  if (engine->s.arithmetic == SDG_FP64) return uploadMfma64(engine->s, host_matrices);
+  if (!matrixCore(engine->s.arithmetic)) return rc;
   return uploadMfma(engine->s, host_matrices);
+}
+
+int sdg_engine_set_kernels(sdg_engine* engine, int family) {
+  if (engine == nullptr || (family != SDG_KERNELS_AUTO && family != SDG_KERNELS_VALU)) {
+    setError("sdg_engine_set_kernels: engine is NULL or the family is unknown");
+    return SDG_ERR_INVALID;
+  }
+  if (family == SDG_KERNELS_VALU && matrixCore(engine->s.arithmetic)) {
+    setError("sdg_engine_set_kernels: the split-16-bit forms run on the matrix cores only");
+    return SDG_ERR_UNSUPPORTED;
+  }
+  engine->s.kernels = family;
+  return SDG_OK;
 }
 
 int sdg_engine_set_layout(sdg_engine* engine, int alignment, int star_format) {
@@ -554,6 +581,7 @@ int sdg_engine_destroy(sdg_engine* engine) {
   if (engine->s.mfmaFrags) SDG_HIP_CHECK(hipFree(engine->s.mfmaFrags));
   if (engine->s.mfmaTables) SDG_HIP_CHECK(hipFree(engine->s.mfmaTables));
   if (engine->s.drMatrices) SDG_HIP_CHECK(hipFree(engine->s.drMatrices));
+  if (engine->s.mfma64Frags) SDG_HIP_CHECK(hipFree(engine->s.mfma64Frags));
   delete engine;
   return SDG_OK;
 }
@@ -605,6 +633,9 @@ int sdg_compute_batched_neighbors_integral(sdg_engine* engine, const sdg_layer* 
   if (engine == nullptr || !validLayer(engine->s, layer, true)) return SDG_ERR_INVALID;
   if (matrixCore(engine->s.arithmetic) && layer->face_traces != nullptr)
     return launchNeighborMfma(engine->s, layer, static_cast<hipStream_t>(stream));
+  // the FP64 engine reads the time-integrated DOFs here, as the reference's
+  // neighbour integral does; its matrix-core neighbour pass runs only inside
+  // the fused proxy step, whose local pass writes the traces it reads
   return launchNeighbor(engine->s, layer, static_cast<hipStream_t>(stream));
 }
 

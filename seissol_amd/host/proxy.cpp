@@ -234,8 +234,11 @@ int run(int argc, char** argv) {
   void* dInt = deviceZeros(n * 9 * B * real);
   const bool traces = arithmetic == Arithmetic::BF16x3 || arithmetic == Arithmetic::BF16x6 ||
                       arithmetic == Arithmetic::F16x3;
+  // the FP64 engine's matrix-core passes (orders 3-7) exchange FP64 face traces
+  const bool traces64 = arithmetic == Arithmetic::FP64 && order >= 3 && order <= 7;
   const int B2P = (B2 + 3) / 4 * 4;  // face rows of the traces, padded to a multiple of 4
-  void* dTraces = traces ? deviceZeros(n * 4 * 9 * B2P * 4) : nullptr;
+  void* dTraces = traces ? deviceZeros(n * 4 * 9 * B2P * 4)
+                         : (traces64 ? deviceZeros(n * 4 * 9 * B2P * 8) : nullptr);
   // dynamic rupture: random DR faces, derivatives and imposed states
   void* dDeriv = nullptr;
   std::vector<void*> drBuffers;

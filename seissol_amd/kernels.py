@@ -90,6 +90,12 @@ class Engine:
         check(lib().sdg_engine_create(C.byref(handle), order, ARITHMETIC[arithmetic]), "sdg_engine_create")
         self.handle = handle
 
+    def set_kernels(self, family: str = "auto"):
+        """Kernel family (``sdg_engine_set_kernels``): "auto" runs the matrix-core
+        passes where the engine has them, "valu" the FP64/FP32 VALU kernels only."""
+        check(lib().sdg_engine_set_kernels(self.handle, {"auto": 0, "valu": 1}[family]), "sdg_engine_set_kernels")
+        self.kernels = family
+
     def set_layout(self, alignment: int = 0, star_format: str = "dense"):
         """Storage layout of the cells' arrays (``sdg_engine_set_layout``)."""
         check(lib().sdg_engine_set_layout(self.handle, int(alignment), STAR_FORMAT[star_format]),

@@ -177,8 +177,13 @@ class DeviceLayer:
         self.time_integrated[:, :, :B] = 0.0
         self.derivatives = (torch.full((n, dreals), fill, dtype=rt, device=dev) if with_derivatives else None)
         B2P = padded_face_basis(setup.order)
-        self.face_traces = (torch.zeros((n, 4, 9, B2P), dtype=torch.float32, device=dev)
-                            if engine.arithmetic in MATRIX_CORE else None)
+        if engine.arithmetic in MATRIX_CORE:
+            self.face_traces = torch.zeros((n, 4, 9, B2P), dtype=torch.float32, device=dev)
+        elif engine.arithmetic == "fp64" and getattr(engine, "kernels", "auto") == "auto" and 3 <= setup.order <= 7:
+            # the FP64 matrix-core passes exchange FP64 face traces
+            self.face_traces = torch.zeros((n, 4, 9, B2P), dtype=torch.float64, device=dev)
+        else:
+            self.face_traces = None
         self.view = Layer(n, ptr(self.dofs), ptr(self.time_integrated), ptr(self.derivatives), ptr(self.star),
                           ptr(self.aplus), ptr(self.aminus), ptr(self.neighbors), ptr(self.face_relations),
                           ptr(self.face_types), ptr(self.face_traces), None, *strides)

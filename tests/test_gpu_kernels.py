@@ -297,3 +297,40 @@ def test_f16x3_stays_within_1e5_of_the_oracle_over_200_steps(gpu):
     per_q = [rel_l2(q[:, p], q_ref[:, p]) for p in range(9)]
     print(f"f16x3 after {steps} steps: rel L2 per quantity max {max(per_q):.2e}")
     assert max(per_q) < 1e-5, per_q
+
+
+@pytest.mark.parametrize("kernels", ["auto", "valu"])
+@pytest.mark.parametrize("order", [3, 4, 5, 6, 7])
+def test_fp64_matrix_core_and_valu_passes_match_oracle(gpu, order, kernels):
+    # "auto": the FP64 ader + local pass on v_mfma_f64_16x16x4 (orders 3-7);
+    # "valu": the VALU kernels; both against the oracle at 1e-12, with DR faces
+    # in the layer (no local or neighbour flux there) and the derivatives written
+    steps = 2
+    setup = make_setup(order, n=700, mesh="proxy", seed=41)
+    setup.face_types[::7, 2] = _capi.SDG_FACE_DYNAMIC_RUPTURE
+    print(f"fp64 {kernels} order={order} n={setup.n}", flush=True)
+    eng = Engine(order, "fp64")
+    eng.set_kernels(kernels)
+    layer = DeviceLayer(eng, setup, with_derivatives=True)
+    gm = global_matrices(order)["flat"]
+    coeffs = monomial_integrate(order, 0.0, setup.dt)
+    star, ap, am = _np(layer.star64), _np(layer.aplus64), _np(layer.aminus64)
+    # one fused ader + local pass: I, derivatives, Q
+    run_proxy(eng, layer, "local", 1, setup.dt)
+    _sync()
+    I_ref, d_ref = oracle.ader(order, gm, coeffs, setup.dofs, star, want_derivatives=True)
+    q_ref = oracle.local(order, gm, I_ref, star, ap, setup.dofs, setup.face_types)
+    assert rel_l2(_np(layer.time_integrated), I_ref) < 1e-12
+    assert rel_l2(_np(layer.derivatives_packed()), d_ref) < 1e-12
+    assert rel_l2(_np(layer.dofs) - setup.dofs, q_ref - setup.dofs) < 1e-11
+    # proxy `all` from the initial state
+    layer.reset_dofs(setup.dofs)
+    run_proxy(eng, layer, "all", steps, setup.dt)
+    _sync()
+    q_ref, I_ref = oracle.proxy_all(order, gm, steps, coeffs, star, ap, am, setup.neighbors, setup.face_relations,
+                                    setup.dofs, setup.face_types)
+    err = rel_l2(_np(layer.dofs), q_ref)
+    print(f"fp64 {kernels} order {order}: rel L2 after {steps} steps {err:.2e}")
+    assert err < 1e-12, err
+    with pytest.raises(_capi.SdgError):
+        Engine(order, "f16x3").set_kernels("valu")
