@@ -79,53 +79,136 @@ __device__ inline void gemm9(doublex4 (&acc)[9], const double* __restrict__ a, i
   }
 }
 
-// y += sum_j (A_j X) star_j for the lane's element: three passes over the
-// input quantity groups {6,7,8}, {0,1,2}, {3,4,5} (the elastic star matrix maps
-// velocities to stresses and stresses to velocities), one matrix at a time
-// with 3 accumulator tiles
-template <int LD>
-__device__ inline void gemmStars(const double* const (&a)[3], int ks, const double* xs, int lane, const double* sS,
-                                 double (&y)[9][4]) {
+// One pass of y += sum_j (A_j X) star_j for the lane's element: the input
+// quantities of group PASS ({6,7,8} -> stresses 0-5, {0,1,2} and {3,4,5} ->
+// velocities 6-8; the elastic star matrix couples velocities and stresses
+// only), one matrix at a time with 3 accumulator tiles
+template <int LD, int PASS>
+__device__ inline void gemmPass(const double* const (&a)[3], int ks, const double* xs, int lane, const double* sS,
+                                double (&y)[9][4]) {
   const int g = lane >> 4, e = lane & 15;
+  constexpr int q0 = PASS == 0 ? 6 : (PASS == 1 ? 0 : 3);
 #pragma unroll
-  for (int pass = 0; pass < 3; ++pass) {
-    const int q0 = pass == 0 ? 6 : (pass == 1 ? 0 : 3);
+  for (int jj = 0; jj < 3; ++jj) {
+    doublex4 acc[3];
 #pragma unroll
-    for (int jj = 0; jj < 3; ++jj) {
-      doublex4 acc[3];
+    for (int i = 0; i < 3; ++i) acc[i] = doublex4{0, 0, 0, 0};
+    const double* aj = a[jj];
+    double an = aj[lane];
+    for (int kk = 0; kk < ks; ++kk) {
+      const double ac = an;
+      if (kk + 1 < ks) an = aj[(kk + 1) * 64 + lane];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) acc[i] = doublex4{0, 0, 0, 0};
-      const double* aj = a[jj];
-      double an = aj[lane];
-      for (int kk = 0; kk < ks; ++kk) {
-        const double ac = an;
-        if (kk + 1 < ks) an = aj[(kk + 1) * 64 + lane];
+      for (int i = 0; i < 3; ++i) acc[i] = mma(ac, xs[((q0 + i) * 16 + e) * LD + 4 * kk + g], acc[i]);
+    }
+    // the star coefficients are read where they are used (else the compiler
+    // keeps all 72 live across the recursion)
+    const double* sSp = sS;
+    asm volatile("" : "+v"(sSp));
 #pragma unroll
-        for (int i = 0; i < 3; ++i) acc[i] = mma(ac, xs[((q0 + i) * 16 + e) * LD + 4 * kk + g], acc[i]);
-      }
-      // the star coefficients are read where they are used (else the compiler
-      // keeps all 72 live across the recursion)
-      const double* sSp = sS;
-      asm volatile("" : "+v"(sSp));
+    for (int p = 0; p < 9; ++p)
 #pragma unroll
-      for (int p = 0; p < 9; ++p)
+      for (int k = 0; k < 3; ++k)
+        if (k < kStarCnt[p]) {
+          const int qi = kStarQ[p][k];
+          if (qi >= q0 && qi < q0 + 3) {
+            const double sv = sSp[jj * 24 + kStarOff[p] + k];
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-          if (k < kStarCnt[p]) {
-            const int qi = kStarQ[p][k];
-            if (qi >= q0 && qi < q0 + 3) {
-              const double sv = sSp[jj * 24 + kStarOff[p] + k];
-#pragma unroll
-              for (int j = 0; j < 4; ++j) y[p][j] += acc[qi - q0][j] * sv;
-            }
+            for (int j = 0; j < 4; ++j) y[p][j] += acc[qi - q0][j] * sv;
           }
-      // finish this contraction before the next product starts: otherwise the
-      // compiler defers all nine contractions past the nine k-loops and keeps
-      // their 27 accumulator tiles live at once
+        }
+    // finish this contraction before the next product starts: otherwise the
+    // compiler defers the contractions past the k-loops and keeps all their
+    // accumulator tiles live at once
 #pragma unroll
-      for (int p = 0; p < 9; ++p)
+    for (int p = (PASS == 0 ? 0 : 6); p < (PASS == 0 ? 6 : 9); ++p)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) asm volatile("" : "+v"(y[p][j]));
+      for (int j = 0; j < 4; ++j) asm volatile("" : "+v"(y[p][j]));
+  }
+}
+
+template <int LD>
+__device__ inline void gemmPassAt(int pass, const double* const (&a)[3], int ks, const double* xs, int lane,
+                                  const double* sS, double (&y)[9][4]) {
+  if (pass == 0) gemmPass<LD, 0>(a, ks, xs, lane, sS, y);
+  else if (pass == 1) gemmPass<LD, 1>(a, ks, xs, lane, sS, y);
+  else gemmPass<LD, 2>(a, ks, xs, lane, sS, y);
+}
+
+// the wave of the pass-1/2 unit k (tile k / 2, pass 1 + k % 2) of a star GEMM
+// over mt m-tiles: after the owners' pass-0 units, round-robin from wave mt on
+constexpr int unitWave(int mt, int k, int waves) { return (mt + k) % waves; }
+// staging slots a star GEMM over mt m-tiles needs (units not run by their owner)
+constexpr int stageSlots(int mt, int waves) {
+  int n = 0;
+  for (int k = 0; k < 2 * mt; ++k) n += unitWave(mt, k, waves) != k / 2 ? 1 : 0;
+  return n;
+}
+template <int O, int WAVES>
+constexpr int maxStageSlots() {
+  int n = stageSlots(cdiv(basis3(O), 16), WAVES);  // the volume
+  for (int d = 1; d < O; ++d) n = n > stageSlots(cdiv(basis3(O - d), 16), WAVES) ? n : stageSlots(cdiv(basis3(O - d), 16), WAVES);
+  return n;
+}
+constexpr int kSlotDoubles = 3 * 4 * 64;  // a lane's 4 rows of the 3 velocities, 64 lanes
+
+// y += sum_j (A_j X) star_j for m-tile w (w < mt) with the 3 mt (tile, pass)
+// units spread over the waves: wave t runs pass 0 of tile t, the pass-1/2
+// units go round-robin from wave mt on; a unit run by a wave other than its
+// tile's owner hands its velocity partial sums over `stage`.  Contains the
+// barrier after which every wave has finished reading X; on return the
+// owner's y holds the whole sum.  a(j, t): the fragments of matrix j, tile t.
+template <int LD, int WAVES, bool BALANCE, typename Frag>
+__device__ inline void starsBalanced(const Frag& fragOf, int mt, int ks, const double* xs, double* stage, int w,
+                                     int lane, const double* sS, double (&y)[9][4]) {
+  if constexpr (!BALANCE) {
+    // every pass of tile w by its owner (where the staging area does not fit)
+    if (w < mt) {
+      const double* a[3] = {fragOf(0, w), fragOf(1, w), fragOf(2, w)};
+      gemmPass<LD, 0>(a, ks, xs, lane, sS, y);
+      gemmPass<LD, 1>(a, ks, xs, lane, sS, y);
+      gemmPass<LD, 2>(a, ks, xs, lane, sS, y);
+    }
+    __syncthreads();
+    return;
+  }
+  if (w < mt) {
+    const double* a[3] = {fragOf(0, w), fragOf(1, w), fragOf(2, w)};
+    gemmPass<LD, 0>(a, ks, xs, lane, sS, y);
+  }
+  int slot = 0;
+  for (int k = 0; k < 2 * mt; ++k) {
+    const int t = k / 2, pass = 1 + k % 2, unitW = unitWave(mt, k, WAVES);
+    if (unitW == w) {
+      const double* a[3] = {fragOf(0, t), fragOf(1, t), fragOf(2, t)};
+      if (unitW == t) {
+        gemmPassAt<LD>(pass, a, ks, xs, lane, sS, y);
+      } else {
+        double tmp[9][4];
+#pragma unroll
+        for (int p = 0; p < 9; ++p)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) tmp[p][j] = 0.0;
+        gemmPassAt<LD>(pass, a, ks, xs, lane, sS, tmp);
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) stage[((slot * 3 + q) * 4 + j) * 64 + lane] = tmp[6 + q][j];
+      }
+    }
+    if (unitW != t) ++slot;
+  }
+  __syncthreads();
+  slot = 0;
+  for (int k = 0; k < 2 * mt; ++k) {
+    const int t = k / 2, unitW = unitWave(mt, k, WAVES);
+    if (unitW != t) {
+      if (w == t)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) y[6 + q][j] += stage[((slot * 3 + q) * 4 + j) * 64 + lane];
+      ++slot;
     }
   }
 }
@@ -137,9 +220,24 @@ __global__ __launch_bounds__(Shape64<O>::THREADS, 1) void kLocalMfma64(
     const double* __restrict__ aplus, const uint8_t* __restrict__ faceTypes, CellLayout lay, LtsOut64 lts) {
   using S = Shape64<O>;
   constexpr int B = S::B, MT = S::MT, LDX = S::LDX, LDW = S::LDW, B2P = S::B2P;
+  // LDS: X, AplusT of the four faces ([element][face][81]), the compact star
+  // matrices ([element][3][24]) and the helper waves' velocity partial sums;
+  // where all four do not fit (ORDER 7) the partial sums share AplusT's area
+  // and AplusT is loaded after the volume product
+  constexpr int kA = kTileElems * 324, kS = kTileElems * 72;
+  constexpr int kBalancedStage = maxStageSlots<O, S::WAVES>() * kSlotDoubles;
+  // the helper waves' partial sums need their own area or AplusT's; without
+  // room for either (ORDER 7) each owner runs all passes of its tile
+  constexpr bool kBalance = (kCols * LDX + kS + (kA > kBalancedStage ? kA : kBalancedStage)) * 8 <= 163840;
+  constexpr int kStage = kBalance ? kBalancedStage : 0;
+  constexpr bool kShareA = (kCols * LDX + kA + kS + kStage) * 8 > 163840;
+  static_assert((kCols * LDX + kS + (kShareA ? (kA > kStage ? kA : kStage) : kA + kStage)) * 8 <= 163840,
+                "LDS of one workgroup per CU");
   __shared__ __attribute__((aligned(16))) double xs[kCols * LDX];
-  __shared__ double sA[kTileElems * 324];   // AplusT of the four faces, [element][face][81]
-  __shared__ double sStar[kTileElems * 72];  // compact star matrices, [element][3][24]
+  __shared__ double sA[kShareA ? (kA > kStage ? kA : kStage) : kA];
+  __shared__ double sStar[kS];
+  __shared__ double sStageOwn[kShareA || kStage == 0 ? 1 : kStage];
+  double* const sStage = kShareA ? sA : sStageOwn;
 
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
@@ -162,12 +260,15 @@ __global__ __launch_bounds__(Shape64<O>::THREADS, 1) void kLocalMfma64(
     const double v = star[el * starStride + (lay.starCompact ? r : 81 * (r / 24) + kStarPos[r % 24])];
     sStar[i] = E0 + s < n ? v : 0.0;
   }
-  for (int i = threadIdx.x; i < kTileElems * 324; i += S::THREADS) {
-    const int s = i / 324, r = i % 324;
-    const int64_t el = E0 + s < n ? E0 + s : n - 1;
-    const double v = aplus[DEF ? el * 324 + r : el * lay.aplusStride + r];
-    sA[i] = E0 + s < n ? v : 0.0;
-  }
+  auto loadAplus = [&]() {
+    for (int i = threadIdx.x; i < kTileElems * 324; i += S::THREADS) {
+      const int s = i / 324, r = i % 324;
+      const int64_t el = E0 + s < n ? E0 + s : n - 1;
+      const double v = aplus[DEF ? el * 324 + r : el * lay.aplusStride + r];
+      sA[i] = E0 + s < n ? v : 0.0;
+    }
+  };
+  if constexpr (!kShareA) loadAplus();
 
   // Q: the lane's rows (zero beyond B and for padding elements); read again
   // for the final update, so only the time integral stays live
@@ -211,10 +312,10 @@ __global__ __launch_bounds__(Shape64<O>::THREADS, 1) void kLocalMfma64(
     for (int p = 0; p < 9; ++p)
 #pragma unroll
       for (int j = 0; j < 4; ++j) dq[p][j] = 0.0;
+    starsBalanced<LDX, S::WAVES, kBalance>(
+        [&](int j, int t) { return frag + S::ck + (static_cast<int64_t>(j) * MT + t) * S::KSB * 64; }, mtiles, ks, xs,
+        sStage, w, lane, sS, dq);  // ends with the barrier after which dQ_{d-1} is no longer read
     if (w < mtiles) {
-      const double* a[3] = {frag + S::ck + (0LL * MT + w) * S::KSB * 64, frag + S::ck + (1LL * MT + w) * S::KSB * 64,
-                            frag + S::ck + (2LL * MT + w) * S::KSB * 64};
-      gemmStars<LDX>(a, ks, xs, lane, sS, dq);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         if (row0 + 4 * j >= m)
@@ -231,7 +332,6 @@ __global__ __launch_bounds__(Shape64<O>::THREADS, 1) void kLocalMfma64(
             if (row0 + 4 * j < m) cellDeriv[doff + p * ldm + row0 + 4 * j] = dq[p][j];
       doff += 9LL * ldm;
     }
-    __syncthreads();  // every wave has read dQ_{d-1}
     if (w < mtiles) {
 #pragma unroll
       for (int p = 0; p < 9; ++p) {
@@ -301,14 +401,16 @@ __global__ __launch_bounds__(Shape64<O>::THREADS, 1) void kLocalMfma64(
     for (int p = 0; p < 9; ++p)
 #pragma unroll
       for (int j = 0; j < 4; ++j) vol[p][j] = 0.0;
-    if (ownsRows) {
-      const double* a[3] = {frag + S::vol + (0LL * MT + w) * S::KSB * 64,
-                            frag + S::vol + (1LL * MT + w) * S::KSB * 64,
-                            frag + S::vol + (2LL * MT + w) * S::KSB * 64};
-      gemmStars<LDX>(a, S::KSB, xs, lane, sS, vol);
-    }
+    starsBalanced<LDX, S::WAVES, kBalance>(
+        [&](int j, int t) { return frag + S::vol + (static_cast<int64_t>(j) * MT + t) * S::KSB * 64; }, MT, S::KSB,
+        xs, sStage, w, lane, sS, vol);
 #pragma unroll
     for (int p = 0; p < 9; ++p) acc[p] = doublex4{vol[p][0], vol[p][1], vol[p][2], vol[p][3]};
+  }
+  if constexpr (kShareA) {
+    __syncthreads();  // the owners have read the staged partial sums
+    loadAplus();
+    __syncthreads();
   }
   // W = traces AplusT_f, each row with its own face's matrix; DR faces give none
   if (ownsTrace) {
