@@ -154,15 +154,13 @@ typedef struct {
   const int32_t* face_neighbors;  /* [n][4], index into time_integrated       */
   const uint8_t* face_relations;  /* [n][4][2]                                */
   const uint8_t* face_types;      /* [n][4] or NULL (all regular)             */
-  void* face_traces;              /* [n][4][9][B2P], B2P = B2 rounded up to a multiple of 4: FP32 for the
-                                     matrix-core forms; FP64 for the FP64 engine (orders 3-7), whose local
-                                     pass then also writes them and whose proxy kernels all/neigh read
-                                     them (its batched neighbour integral always reads time_integrated) */
-                                  /* to a multiple of 4 (padding rows zero):  */
-                                  /* rT_f I of every face;                    */
-                                  /* required by the matrix-core arithmetics  */
-                                  /* SDG_BF16X3/X6, SDG_F16X3 (written by the */
-                                  /* local pass, read by the neighbour pass)  */
+  void* face_traces;              /* [n][4][9][B2P]: rT_f I of every face, B2P = B2 rounded up to a
+                                     multiple of 4 (padding rows zero).  Required by the matrix-core
+                                     arithmetics SDG_BF16X3/X6, SDG_F16X3 (FP32; written by the local
+                                     pass, read by the neighbour pass).  Optional for the FP64 engine
+                                     at orders 3-7 (FP64): its local pass then also writes them and
+                                     the proxy kernels all/neigh read them; its batched neighbour
+                                     integral always reads time_integrated */
   const uint16_t* lts_setup;      /* [n] LTS bits (SDG_LTS_*) or NULL: global */
                                   /* time stepping                            */
   int64_t star_stride;            /* reals between cells' star matrices, 0: dense */

@@ -301,13 +301,22 @@ __device__ inline void gemmTile(floatx4 (&acc)[JJ][NQ], const uint16_t* const (&
 // The lane's four consecutive basis rows row0..row0+3 of the 9 quantities of
 // its element at q[p * ld + j] (row0 = 16 w + 4 g): one 16-byte access per
 // quantity when V4 (ld a multiple of 4, the block 16-byte aligned), else per
-// value; rows from B on (the padding of an aligned layout) are never read.
+// value; rows from B on (the padding of an aligned layout) are never used.
+// V4: qSafe is an in-bounds quad of the same quantity block (the lane's own
+// when it is real): every load is issued unconditionally and the value
+// selected after, since a load under a branch makes the compiler wait on it
+// before the next one.
 template <int B, bool V4>
-__device__ inline void loadRows(const float* q, int ld, int row0, bool valid, float (&v)[9][4]) {
+__device__ inline void loadRows(const float* q, const float* qSafe, int ld, int row0, bool valid,
+                                float (&v)[9][4]) {
   if constexpr (V4) {
+    const bool live = valid && row0 < B;
+    float4 raw[9];
+#pragma unroll
+    for (int p = 0; p < 9; ++p) raw[p] = *reinterpret_cast<const float4*>(qSafe + p * ld);
 #pragma unroll
     for (int p = 0; p < 9; ++p) {
-      float4 x = (valid && row0 < B) ? *reinterpret_cast<const float4*>(q + p * ld) : float4{0, 0, 0, 0};
+      float4 x = live ? raw[p] : float4{0, 0, 0, 0};
       if constexpr (B % 4 != 0) {
         if (row0 + 1 >= B) x.y = 0.0f;
         if (row0 + 2 >= B) x.z = 0.0f;
@@ -453,13 +462,23 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     // the first store (clamped addresses, in bounds: a launched tile has E0 < n)
     constexpr int SI = cdiv(kTileElems * 72, S::THREADS);
     float sv[SI];
+    const int64_t starStride = DEF ? (lay.starCompact ? 72 : 243) : lay.starStride;
+    if (lay.starCompact) {  // wave-uniform: the compact format needs no position table
 #pragma unroll
-    for (int u = 0; u < SI; ++u) {
-      const int i = threadIdx.x + u * S::THREADS;
-      const int s = min(i / 72, kTileElems - 1), r = i % 72;
-      const int64_t el = E0 + s < n ? E0 + s : n - 1;
-      const int64_t starStride = DEF ? (lay.starCompact ? 72 : 243) : lay.starStride;
-      sv[u] = star[el * starStride + (lay.starCompact ? r : 81 * (r / 24) + kStarPos[r % 24])];
+      for (int u = 0; u < SI; ++u) {
+        const int i = threadIdx.x + u * S::THREADS;
+        const int s = min(i / 72, kTileElems - 1), r = i % 72;
+        const int64_t el = E0 + s < n ? E0 + s : n - 1;
+        sv[u] = star[el * starStride + r];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < SI; ++u) {
+        const int i = threadIdx.x + u * S::THREADS;
+        const int s = min(i / 72, kTileElems - 1), r = i % 72;
+        const int64_t el = E0 + s < n ? E0 + s : n - 1;
+        sv[u] = star[el * starStride + 81 * (r / 24) + kStarPos[r % 24]];
+      }
     }
 #pragma unroll
     for (int u = 0; u < SI; ++u) {
@@ -470,7 +489,13 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
 
   // Q itself is read again at the end, so only the time integral stays live
   float q[1][9][4], iacc[1][9][4];
-  loadRows<B, V4>(tileDofs + laneQ, ldq, 16 * w + 4 * g, valid, q[0]);
+  {
+    // in-bounds quad for lanes without real rows: the last quad of a real
+    // element's block (V4: ldq a multiple of 4)
+    const int64_t safeElem = valid ? elem : n - 1;
+    const int safeRow = V4 ? min(16 * w + 4 * g, (B - 1) / 4 * 4) : 0;
+    loadRows<B, V4>(tileDofs + laneQ, dofs + safeElem * qStride + safeRow, ldq, 16 * w + 4 * g, valid, q[0]);
+  }
   // LTS: which of buffer and derivatives this cell provides
   [[maybe_unused]] unsigned setup = 0;
   [[maybe_unused]] bool storeDeriv = false;
@@ -809,21 +834,38 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
   float* const tileDofs = dofs + E0 * qStride;
   const int laneQ = e * static_cast<int>(qStride) + 16 * w + 4 * g;
 
-  for (int i = threadIdx.x; i < 3 * B2 * O; i += blockDim.x) {
-    sFP[i] = tables[S::tFP + i];
-    sFPCol[i] = static_cast<int>(tables[S::tFPCol + i]);
-  }
-  // neighbour links of the four faces, read once
-  if (threadIdx.x < kTileElems * 4) {
-    const int s = threadIdx.x >> 2, f = threadIdx.x & 3;
-    const int64_t el = E0 + s;
-    int link = -1, h = 0;
-    if (el < n && (faceTypes == nullptr || faceTypes[el * 4 + f] == SDG_FACE_REGULAR)) {
-      link = neighbors[el * 4 + f] * 4 + min(static_cast<int>(faceRel[(el * 4 + f) * 2 + 0]), 3);
-      h = min(static_cast<int>(faceRel[(el * 4 + f) * 2 + 1]), 2);
+  // the fP tables and the neighbour links of the four faces, read once; every
+  // load is issued unconditionally from an in-bounds (clamped) address and its
+  // value selected after, so that all of them are in flight together (a load
+  // under a branch makes the compiler wait on it before the next)
+  {
+    constexpr int FPN = 3 * B2 * O, FPI = cdiv(FPN, S::THREADS);
+    float fv[FPI], fc[FPI];
+#pragma unroll
+    for (int u = 0; u < FPI; ++u) {
+      const int i = min(static_cast<int>(threadIdx.x) + u * S::THREADS, FPN - 1);
+      fv[u] = tables[S::tFP + i];
+      fc[u] = tables[S::tFPCol + i];
     }
-    sLink[threadIdx.x] = link;
-    sRel[threadIdx.x] = h;
+    static_assert(kTileElems * 4 <= S::THREADS, "one thread per (element, face) link");
+    const int s = threadIdx.x >> 2 & (kTileElems - 1), f = threadIdx.x & 3;
+    const int64_t el = E0 + s < n ? E0 + s : n - 1;
+    const int type = faceTypes != nullptr ? faceTypes[el * 4 + f] : SDG_FACE_REGULAR;
+    const int nb = neighbors[el * 4 + f];
+    const int side = faceRel[(el * 4 + f) * 2 + 0], rel = faceRel[(el * 4 + f) * 2 + 1];
+#pragma unroll
+    for (int u = 0; u < FPI; ++u) {
+      const int i = threadIdx.x + u * S::THREADS;
+      if (i < FPN) {
+        sFP[i] = fv[u];
+        sFPCol[i] = static_cast<int>(fc[u]);
+      }
+    }
+    if (threadIdx.x < kTileElems * 4) {
+      const bool use = E0 + s < n && type == SDG_FACE_REGULAR;
+      sLink[threadIdx.x] = use ? nb * 4 + min(side, 3) : -1;
+      sRel[threadIdx.x] = use ? min(rel, 2) : 0;
+    }
   }
   // rows [B2P, 32) of the operand stay zero (rows [B2, B2P) are written as zeros by the items)
   if constexpr (B2P < 32) {
