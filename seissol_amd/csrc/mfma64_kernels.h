@@ -251,21 +251,49 @@ __global__ __launch_bounds__(Shape64<O>::THREADS, 1) void kLocalMfma64(
   const bool ownsRows = w < MT;  // waves beyond MT own only a trace tile
   double* const laneDofs = dofs + (valid ? elem : 0) * qStride + (ownsRows ? row0 : 0);
 
-  // the tile's compact star matrices and AplusT (clamped addresses, in bounds:
-  // a launched tile has E0 < n)
-  for (int i = threadIdx.x; i < kTileElems * 72; i += S::THREADS) {
-    const int s = i / 72, r = i % 72;
-    const int64_t el = E0 + s < n ? E0 + s : n - 1;
+  // the tile's compact star matrices and AplusT, staged through registers: every
+  // load of a thread is issued before the first store, from a clamped address
+  // (in bounds: a launched tile has E0 < n), since a loop that stores each
+  // value as it arrives waits on every load in turn
+  {
+    constexpr int SPT = cdiv(kTileElems * 72, S::THREADS);
     const int64_t starStride = DEF ? (lay.starCompact ? 72 : 243) : lay.starStride;
-    const double v = star[el * starStride + (lay.starCompact ? r : 81 * (r / 24) + kStarPos[r % 24])];
-    sStar[i] = E0 + s < n ? v : 0.0;
+    double sv[SPT];
+    if (lay.starCompact) {  // wave-uniform: the compact format needs no position table
+#pragma unroll
+      for (int u = 0; u < SPT; ++u) {
+        const int i = min(static_cast<int>(threadIdx.x) + u * S::THREADS, kTileElems * 72 - 1);
+        const int64_t el = E0 + i / 72 < n ? E0 + i / 72 : n - 1;
+        sv[u] = star[el * starStride + i % 72];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < SPT; ++u) {
+        const int i = min(static_cast<int>(threadIdx.x) + u * S::THREADS, kTileElems * 72 - 1);
+        const int64_t el = E0 + i / 72 < n ? E0 + i / 72 : n - 1;
+        const int r = i % 72;
+        sv[u] = star[el * starStride + 81 * (r / 24) + kStarPos[r % 24]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SPT; ++u) {
+      const int i = threadIdx.x + u * S::THREADS;
+      if (i < kTileElems * 72) sStar[i] = E0 + i / 72 < n ? sv[u] : 0.0;
+    }
   }
   auto loadAplus = [&]() {
-    for (int i = threadIdx.x; i < kTileElems * 324; i += S::THREADS) {
-      const int s = i / 324, r = i % 324;
-      const int64_t el = E0 + s < n ? E0 + s : n - 1;
-      const double v = aplus[DEF ? el * 324 + r : el * lay.aplusStride + r];
-      sA[i] = E0 + s < n ? v : 0.0;
+    constexpr int APT = cdiv(kTileElems * 324, S::THREADS);
+    double av[APT];
+#pragma unroll
+    for (int u = 0; u < APT; ++u) {
+      const int i = min(static_cast<int>(threadIdx.x) + u * S::THREADS, kTileElems * 324 - 1);
+      const int64_t el = E0 + i / 324 < n ? E0 + i / 324 : n - 1;
+      av[u] = aplus[DEF ? el * 324 + i % 324 : el * lay.aplusStride + i % 324];
+    }
+#pragma unroll
+    for (int u = 0; u < APT; ++u) {
+      const int i = threadIdx.x + u * S::THREADS;
+      if (i < kTileElems * 324) sA[i] = E0 + i / 324 < n ? av[u] : 0.0;
     }
   };
   if constexpr (!kShareA) loadAplus();
@@ -497,20 +525,36 @@ __global__ __launch_bounds__(Shape64<O>::NTHREADS, 2) void kNeighborMfma64(
   const int64_t qStride = DEF ? 9 * B : lay.qStride;
   const int row0 = 16 * w + g;  // the lane's rows row0 + 4 r
 
-  for (int i = threadIdx.x; i < 3 * B2 * O; i += THREADS) {
-    sFP[i] = frag[S::tFP + i];
-    sFPCol[i] = static_cast<int>(frag[S::tFPCol + i]);
-  }
-  if (threadIdx.x < kTileElems * 4) {
-    const int s = threadIdx.x >> 2, f = threadIdx.x & 3;
-    const int64_t el = E0 + s;
-    int link = -1, h = 0;
-    if (el < n && (faceTypes == nullptr || faceTypes[el * 4 + f] == SDG_FACE_REGULAR)) {
-      link = neighbors[el * 4 + f] * 4 + min(static_cast<int>(faceRel[(el * 4 + f) * 2 + 0]), 3);
-      h = min(static_cast<int>(faceRel[(el * 4 + f) * 2 + 1]), 2);
+  // the fP tables and the neighbour links, every load issued unconditionally
+  // from an in-bounds (clamped) address before the first store
+  {
+    constexpr int FPN = 3 * B2 * O, FPI = cdiv(FPN, THREADS);
+    double fv[FPI], fc[FPI];
+#pragma unroll
+    for (int u = 0; u < FPI; ++u) {
+      const int i = min(static_cast<int>(threadIdx.x) + u * THREADS, FPN - 1);
+      fv[u] = frag[S::tFP + i];
+      fc[u] = frag[S::tFPCol + i];
     }
-    sLink[threadIdx.x] = link;
-    sRel[threadIdx.x] = h;
+    static_assert(kTileElems * 4 <= THREADS, "one thread per (element, face) link");
+    const int s = threadIdx.x >> 2 & (kTileElems - 1), f = threadIdx.x & 3;
+    const int64_t el = E0 + s < n ? E0 + s : n - 1;
+    const int type = faceTypes != nullptr ? faceTypes[el * 4 + f] : SDG_FACE_REGULAR;
+    const int nb = neighbors[el * 4 + f];
+    const int side = faceRel[(el * 4 + f) * 2 + 0], rel = faceRel[(el * 4 + f) * 2 + 1];
+#pragma unroll
+    for (int u = 0; u < FPI; ++u) {
+      const int i = threadIdx.x + u * THREADS;
+      if (i < FPN) {
+        sFP[i] = fv[u];
+        sFPCol[i] = static_cast<int>(fc[u]);
+      }
+    }
+    if (threadIdx.x < kTileElems * 4) {
+      const bool use = E0 + s < n && type == SDG_FACE_REGULAR;
+      sLink[threadIdx.x] = use ? nb * 4 + min(side, 3) : -1;
+      sRel[threadIdx.x] = use ? min(rel, 2) : 0;
+    }
   }
   __syncthreads();
   double2 gv[GPT];
