@@ -65,6 +65,24 @@ __device__ inline void storeX(double* xs, int col, int base, int rows, const dou
     if (base + 4 * r < rows) d[4 * r] = v[r];
 }
 
+// q[p * ld + row0 + 4 j] += acc[p][j] for the lane's rows below B: all 36 old
+// values are loaded first, unconditionally from in-bounds (clamped) rows, then
+// stored where real (a read-modify-write under a branch waits on every load)
+template <int B>
+__device__ inline void addRows(double* q, int ld, int row0, bool live, const doublex4 (&acc)[9]) {
+  double old[9][4];
+#pragma unroll
+  for (int p = 0; p < 9; ++p)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) old[p][j] = q[p * ld + min(row0 + 4 * j, B - 1)];
+  if (live)
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (row0 + 4 * j < B) q[p * ld + row0 + 4 * j] = old[p][j] + acc[p][j];
+}
+
 // acc[q] += A_tile X(:, quantity q) over `ksteps` k-steps, q < 9
 template <int LD>
 __device__ inline void gemm9(doublex4 (&acc)[9], const double* __restrict__ a, int ksteps, const double* xs,
@@ -479,12 +497,7 @@ __global__ __launch_bounds__(Shape64<O>::THREADS, 1) void kLocalMfma64(
     if (ownsRows)
       gemm9<LDW>(acc, frag + S::rdm + (static_cast<int64_t>(w) * 4 + f) * S::KSF * 64, S::KSF, xs, lane);
   }
-  if (valid && ownsRows)
-#pragma unroll
-    for (int p = 0; p < 9; ++p)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (row0 + 4 * j < B) laneDofs[p * ldq + 4 * j] += acc[p][j];
+  addRows<B>(dofs + (valid ? elem : 0) * qStride, ldq, row0, valid && ownsRows, acc);
 }
 
 // Neighbour flux from the neighbours' face traces (Neighbor.cpp:66-109; spec
@@ -642,14 +655,7 @@ __global__ __launch_bounds__(Shape64<O>::NTHREADS, 2) void kNeighborMfma64(
     __syncthreads();
     gemm9<LDW>(acc, frag + S::rdmN + (static_cast<int64_t>(w) * 4 + f) * S::KSF * 64, S::KSF, xs, lane);
   }
-  if (valid) {
-    double* const laneDofs = dofs + (E0 + e) * qStride + row0;
-#pragma unroll
-    for (int p = 0; p < 9; ++p)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (row0 + 4 * j < B) laneDofs[p * ldq + 4 * j] += acc[p][j];
-  }
+  addRows<B>(dofs + (valid ? E0 + e : 0) * qStride, ldq, row0, valid, acc);
 }
 
 template <int B>
