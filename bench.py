@@ -4,7 +4,8 @@ Workload (default): elastic ORDER 6 (56 basis functions x 9 quantities),
 1,000,000 tetrahedra of the reference proxy's layer (seeded random affine
 tetrahedra, 4 random face neighbours per element with random sides and
 orientations, periodic wrap), star matrices stored as their 24 non-zeros,
-rho ~ U[2000, 3000], vp 6000, vs 3464, DOFs ~ U[-1, 1], dt from CFL 0.5,
+rho ~ U[2000, 3000], vp 6000, vs 3464, DOFs ~ U[-1, 1] (x 2^-32 on the proxy
+mesh, see DOF_SCALE_PROXY), dt from CFL 0.5,
 global time stepping, 200 steps, in the chosen split-fp16 matrix-core form;
 one step = ader + local integral + neighbour integral of every element.
 Inputs are resident in HBM before timing starts.
@@ -30,6 +31,7 @@ import numpy as np
 
 METRIC = "element-updates/s ORDER=6 elastic, chosen vs plain FP64 GPU at L2 err <=1e-5"
 HBM_PEAK_GBS = 8000.0
+DOF_SCALE_PROXY = 2.0 ** -32  # initial DOFs on the proxy mesh: U[-1, 1) x this
 
 
 def log(msg: str) -> None:
@@ -96,7 +98,7 @@ def measured_traffic(form: str, order: int, elements: int, kernel: str, mesh: st
 
 
 def measured_mfma(form: str, order: int, elements: int, kernel: str, mesh: str, star: str):
-    """fp16/bf16 matrix-core FLOP per launch and MFMA busy fraction of `kernel`
+    """Matrix-core FLOP per launch and MFMA busy fraction of `kernel`
     from a counter summary under profiles/ (a line `bench-mfma: form=F order=O
     elements=N mesh=M star=S kernel=K flop=X busy_frac=Y`), or None."""
     from pathlib import Path
@@ -345,6 +347,14 @@ def main(argv=None) -> int:
     dims = known.get(args.elements) if args.mesh == "periodic" else None
     t0 = time.perf_counter()
     setup = make_setup(args.order, n=args.elements, mesh=args.mesh, seed=args.seed, dims=dims)
+    if args.mesh == "proxy":
+        # The proxy's random face links make an inconsistent mesh on which the
+        # solution grows (about 2x every 10 steps after a first jump); random
+        # DOFs in [-2^-32, 2^-32) keep the FP32 forms' CK derivatives (some |Q| / dt^5
+        # in size) inside FP32 range over the 200 steps, so that the run stays
+        # finite and is compared with FP64 to its end.  The values do not
+        # change the work.
+        setup.dofs *= DOF_SCALE_PROXY
     log(f"setup {setup.n} elements ({args.mesh}) in {time.perf_counter() - t0:.1f} s, dt={setup.dt:.3e}")
     eng = Engine(args.order, args.form)
     eng.set_layout(0, args.star)
@@ -424,7 +434,7 @@ def main(argv=None) -> int:
             return points, final, ms64 / done
 
         points, final, ms64 = lockstep(setup, layer, eng, 10)
-        plain = {"value": setup.n / (ms64 / 1e3), "ms_per_step": ms64, "form": "fp64 (VALU, the drop-in)",
+        plain = {"value": setup.n / (ms64 / 1e3), "ms_per_step": ms64, "form": "fp64 (the FP64 drop-in: matrix-core local pass, FP64 traces)",
                  "mesh": args.mesh}
         last_ok = [p for p in points if p["finite"]]
         log(f"plain FP64: {plain['value'] / 1e6:.2f} M element-updates/s; {args.mesh} mesh: finite through step "
@@ -484,7 +494,10 @@ def main(argv=None) -> int:
                      "traffic": traffic, "bytes_per_element": ab[dom],
                      "launch_ms": per_kernel[dom], "other_launch_ms": {k: v for k, v in per_kernel.items() if k != dom},
                      "matrix_core": {
-                         "peak_tflops": 2500.0 if args.form in ("bf16x3", "bf16x6", "f16x3") else None,
+                         # dense peaks: fp16/bf16 2.5 PFLOP/s; FP64 78.6 TFLOP/s (the FP64 engine's
+                         # matrix-core pass, orders 3-7); the FP32 form runs on the VALU
+                         "peak_tflops": 2500.0 if args.form in ("bf16x3", "bf16x6", "f16x3")
+                         else (78.6 if args.form == "fp64" and 3 <= args.order <= 7 else None),
                          # counter-measured (rocprofv3 MFMA_MOPS x 512 per launch) over this run's launch time
                          "achieved_tflops": (mfma["flop"] / (per_kernel[dom] / 1e3) / 1e12) if mfma else None,
                          "busy_frac": mfma["busy_frac"] if mfma else None,
