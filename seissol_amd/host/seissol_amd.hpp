@@ -84,6 +84,14 @@ class GlobalData {
   void setMatrices(const std::vector<double>& flat) {
     check(sdg_engine_set_global_matrices(engine_, flat.data()), "sdg_engine_set_global_matrices");
   }
+  /// Storage layout of the cells' arrays (the reference fixes it at build time:
+  /// 128-byte alignment gives [9][64] DOF blocks at ORDER 6) and star format
+  /// (SDG_STAR_DENSE / SDG_STAR_COMPACT).
+  void setLayout(int alignment, int starFormat) {
+    check(sdg_engine_set_layout(engine_, alignment, starFormat), "sdg_engine_set_layout");
+  }
+  /// Kernel family: SDG_KERNELS_AUTO (matrix cores where available) or SDG_KERNELS_VALU.
+  void setKernels(int family) { check(sdg_engine_set_kernels(engine_, family), "sdg_engine_set_kernels"); }
   sdg_engine* engine() const { return engine_; }
   int order() const { return order_; }
   Arithmetic arithmetic() const { return arithmetic_; }
