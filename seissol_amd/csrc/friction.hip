@@ -34,6 +34,13 @@ constexpr double kMovedDecades = 0.30103;  // log10(2)
 constexpr double kJumpDecades = 1.0;
 constexpr double kLn10 = 2.30258509299404568402;
 
+// The network's forward pass, kept out of line: inlined into kFriction, its
+// unrolled products and LDS weight loads pushed the kernel past its registers
+// (916 B of spills per lane); as a call it costs the warm-start branch only.
+__device__ __noinline__ float mlpEvaluateCall(const float* prm, const float (&x)[kMlpInputs]) {
+  return mlpEvaluate(prm, x);
+}
+
 // natural log of the closed-form creep root V_c = 2 V0 exp(-psi/a) sinh(S/N),
 // which solves N asinh(V exp(psi/a) / (2 V0)) = S; -inf when S or N is not positive
 __device__ inline double creepLogRoot(double S, double N, double psiOverA, double sr0) {
@@ -204,7 +211,7 @@ __global__ __launch_bounds__(256) void kFriction(sdg_friction_params prm, sdg_fa
         } else if (useMlp && !creeping) {
           base = mlpFeatures(lvc, upper, nScale, sv / pp.a, vPrev, pp.a, feat);
           // the root lies in (0, min(V_c, S)]
-          const double lg = fmin(base + static_cast<double>(mlpEvaluate(sMlp, feat)), fmin(lvc / kLn10, log10(upper)));
+          const double lg = fmin(base + static_cast<double>(mlpEvaluateCall(sMlp, feat)), fmin(lvc / kLn10, log10(upper)));
           if (isfinite(lg) && fabs(lg - log10(vPrev)) > kJumpDecades) test = fmax(kAlmostZero, pow(10.0, lg));
         }
       }
