@@ -297,7 +297,7 @@ def main(argv=None) -> int:
                     help="star matrices as 3 x 81 dense or 3 x 24 non-zeros (the reference's sparse GPU layout)")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--cpu-elements", type=int, default=100_000)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error-check", action="store_true",
                     help="skip the rel. L2 comparison of the form against the plain FP64 path")
