@@ -382,15 +382,22 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
   }
 
   if (storeBuffer) {
+    real* const elemInteg = integ + e * lay.qStride;
+    // accumulated buffers: all old values loaded first (clamped rows, in
+    // bounds), then stored; a read-modify-write per value through runtime
+    // strides waits on every load in turn
+    real old[R][9] = {};
+    if (accumulate)
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int p = 0; p < 9; ++p) old[r][p] = elemInteg[p * lay.ldq + min(t + r * TPE, B - 1)];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int row = t + r * TPE;
       if (row < B)
 #pragma unroll
-        for (int p = 0; p < 9; ++p) {
-          real* b = integ + e * lay.qStride + p * lay.ldq + row;
-          *b = accumulate ? *b + iacc[r][p] : iacc[r][p];
-        }
+        for (int p = 0; p < 9; ++p) elemInteg[p * lay.ldq + row] = old[r][p] + iacc[r][p];
     }
   }
   if constexpr (WithLocal) {
@@ -694,12 +701,19 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
     for (int p = 0; p < 9; ++p) acc[0][r][p] = real(0);
   gemmRows<1, R, kProd>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
   if (active) {
+    // all old values loaded first (clamped rows, in bounds), then stored
+    real* const q = dofs + e * lay.qStride;
+    real old[R][9];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int p = 0; p < 9; ++p) old[r][p] = q[p * lay.ldq + min(t + r * TPE, B - 1)];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int row = t + r * TPE;
       if (row < B)
 #pragma unroll
-        for (int p = 0; p < 9; ++p) dofs[e * lay.qStride + p * lay.ldq + row] += acc[0][r][p];
+        for (int p = 0; p < 9; ++p) q[p * lay.ldq + row] = old[r][p] + acc[0][r][p];
     }
   }
 }

@@ -397,16 +397,22 @@ __global__ __launch_bounds__(Shape64<O>::THREADS, 1) void kLocalMfma64(
     accumulate = lts.setup != nullptr && !lts.reset && (setup & SDG_LTS_ACCUMULATE_BUFFERS);
   }
   if (storeBuffer && ownsRows) {
-    double* const laneInteg = integ + elem * qStride + row0;
+    double* const elemInteg = integ + elem * qStride;
+    // accumulated buffers: all old values loaded first (clamped rows, in
+    // bounds), then stored, so that the loads are in flight together
+    double old[9][4] = {};
+    if constexpr (LTS) {
+      if (accumulate)
+#pragma unroll
+        for (int p = 0; p < 9; ++p)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) old[p][j] = elemInteg[p * ldq + min(row0 + 4 * j, B - 1)];
+    }
 #pragma unroll
     for (int p = 0; p < 9; ++p)
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (row0 + 4 * j < B) {
-          double* b = laneInteg + p * ldq + 4 * j;
-          if constexpr (LTS) *b = accumulate ? *b + iacc[p][j] : iacc[p][j];
-          else *b = iacc[p][j];
-        }
+        if (row0 + 4 * j < B) elemInteg[p * ldq + row0 + 4 * j] = old[p][j] + iacc[p][j];
   }
   if (ownsRows)
 #pragma unroll

@@ -630,18 +630,23 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     accumulate = lts.setup != nullptr && !lts.reset && (setup & SDG_LTS_ACCUMULATE_BUFFERS);
   }
   if (storeBuffer) {
-    float* const tileInteg = integ + E0 * qStride;
+    float* const elemInteg = integ + elem * qStride;
+    const int row0 = 16 * w + 4 * g;
+    // accumulated buffers: all old values loaded first (clamped rows, in
+    // bounds), then stored, so that the loads are in flight together
+    float old[9][4] = {};
+    if constexpr (LTS) {
+      if (accumulate)
+#pragma unroll
+        for (int p = 0; p < 9; ++p)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) old[p][j] = elemInteg[p * ldq + min(row0 + j, B - 1)];
+    }
 #pragma unroll
     for (int p = 0; p < 9; ++p)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int row = 16 * w + 4 * g + j;
-        if (row < B) {
-          float* b = tileInteg + laneQ + p * ldq + j;
-          if constexpr (LTS) *b = accumulate ? *b + iacc[0][p][j] : iacc[0][p][j];
-          else *b = iacc[0][p][j];
-        }
-      }
+      for (int j = 0; j < 4; ++j)
+        if (row0 + j < B) elemInteg[p * ldq + row0 + j] = old[p][j] + iacc[0][p][j];
   }
   if constexpr (scaled) {
     waveColumnMax<1>(iacc, true, sMax, w, lane);
