@@ -151,12 +151,17 @@ __global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, c
     // the element's derivatives in chunks of six loads per lane, each chunk
     // issued before its first value is stored (clamped addresses, in bounds)
     constexpr int DI = (DS + 63) / 64;
+    // the gather offsets are recomputed per face (an opaque copy of the lane
+    // index): hoisted out of the face loop they are spilled, and every load
+    // then waits on the reload of its address
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
 #pragma unroll
     for (int c = 0; c < DI; c += 6) {
       real tmp[6];
 #pragma unroll
       for (int u = 0; u < 6; ++u) {
-        const int i = min(lane + 64 * (c + u), DS - 1);
+        const int i = min(ln + 64 * (c + u), DS - 1);
         tmp[u] = deriv[e * lay.dsize + derivativeOffset<O>(i, lay)];
       }
 #pragma unroll
@@ -268,6 +273,9 @@ __global__ __launch_bounds__(kDrThreads) void kDrNodalFlux(int64_t n, const real
     for (int r = 0; r < (B + kDrThreads - 1) / kDrThreads; ++r) {
       const int k = tid + r * kDrThreads;
       if (k < B)
+        // unrolled so that several loads of W are in flight at once (rolled,
+        // each iteration waited on its load)
+#pragma unroll 7
         for (int l = 0; l < NP; ++l) {
           const real w = W[k * NP + l];
 #pragma unroll
@@ -276,11 +284,16 @@ __global__ __launch_bounds__(kDrThreads) void kDrNodalFlux(int64_t n, const real
     }
   }
   if (!any) return;
+  // all old values loaded first (clamped rows, in bounds), then stored
+  real* const q = dofs + e * lay.qStride;
   for (int r = 0; r < (B + kDrThreads - 1) / kDrThreads; ++r) {
     const int k = tid + r * kDrThreads;
+    real old[9];
+#pragma unroll
+    for (int p = 0; p < 9; ++p) old[p] = q[p * lay.ldq + min(k, B - 1)];
     if (k < B)
 #pragma unroll
-      for (int p = 0; p < 9; ++p) dofs[e * lay.qStride + p * lay.ldq + k] += acc[r][p];
+      for (int p = 0; p < 9; ++p) q[p * lay.ldq + k] = old[p] + acc[r][p];
   }
 }
 
