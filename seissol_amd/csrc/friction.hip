@@ -171,6 +171,12 @@ __global__ __launch_bounds__(256) void kFriction(sdg_friction_params prm, sdg_fa
   double srm = L.slip_rate_magnitude[k];
   double muv = L.mu[k];
   uint8_t convInner = L.convergence_inner[k], convOuter = L.convergence_outer[k];
+  // the per-point outputs accumulated over the time points, read once here and
+  // written once after the time loop (a read-modify-write per time point
+  // waited on each load in turn)
+  double accSlip = L.accumulated_slip[k], slip1 = L.slip1[k], slip2 = L.slip2[k];
+  double peakSr = L.peak_slip_rate[k], ruptureTime = L.rupture_time[k], dynStressTime = L.dyn_stress_time[k];
+  uint8_t rupturePending = L.rupture_time_pending[k], dynStressPending = L.dyn_stress_time_pending[k];
   int newtonEvals = 0, newtonSteps = 0;
   const bool closedForm = opt.closed_form_start != 0;
 
@@ -271,20 +277,30 @@ __global__ __launch_bounds__(256) void kFriction(sdg_friction_params prm, sdg_fa
       slipRate2 = srm * t2 / divisor;
       trT1[o] = fsT1[o] - etaS * slipRate1;
       trT2[o] = fsT2[o] - etaS * slipRate2;
-      L.accumulated_slip[k] += srm * dt;
-      L.slip1[k] += slipRate1 * dt;
-      L.slip2[k] += slipRate2 * dt;
-      if (L.rupture_time_pending[k] && srm > 0.001) {
-        L.rupture_time[k] = startTime;
-        L.rupture_time_pending[k] = 0;
+      accSlip += srm * dt;
+      slip1 += slipRate1 * dt;
+      slip2 += slipRate2 * dt;
+      if (rupturePending && srm > 0.001) {
+        ruptureTime = startTime;
+        rupturePending = 0;
       }
-      const double rt = L.rupture_time[k];
-      if (rt > 0.0 && rt <= startTime && L.dyn_stress_time_pending[k] && muv <= pp.muw + 0.05 * (pp.f0 - pp.muw)) {
-        L.dyn_stress_time[k] = startTime;
-        L.dyn_stress_time_pending[k] = 0;
+      if (ruptureTime > 0.0 && ruptureTime <= startTime && dynStressPending &&
+          muv <= pp.muw + 0.05 * (pp.f0 - pp.muw)) {
+        dynStressTime = startTime;
+        dynStressPending = 0;
       }
-      L.peak_slip_rate[k] = fmax(L.peak_slip_rate[k], srm);
+      peakSr = fmax(peakSr, srm);
     }
+  }
+  if (active) {
+    L.accumulated_slip[k] = accSlip;
+    L.slip1[k] = slip1;
+    L.slip2[k] = slip2;
+    L.peak_slip_rate[k] = peakSr;
+    L.rupture_time[k] = ruptureTime;
+    L.rupture_time_pending[k] = rupturePending;
+    L.dyn_stress_time[k] = dynStressTime;
+    L.dyn_stress_time_pending[k] = dynStressPending;
   }
   // state resampling over the face's points (postHook)
   __syncthreads();
