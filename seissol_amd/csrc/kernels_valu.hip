@@ -600,37 +600,60 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
   real* xu = sU + sl * B2 * 9;
   real* xb = sB + sl * S::FS;
 
+  // the four faces' links, read once (clamped: off regular faces nb = 0)
+  int64_t nbs[4];
+  bool regs[4];
+  uint8_t rels[4][2];
+  const int64_t ec = active ? e : 0;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    regs[f] = active && (faceTypes == nullptr || faceTypes[ec * 4 + f] == SDG_FACE_REGULAR);
+    const int32_t nbRaw = neighbors[ec * 4 + f];
+    nbs[f] = regs[f] ? nbRaw : 0;
+    rels[f][0] = faceRel[(ec * 4 + f) * 2 + 0];
+    rels[f][1] = faceRel[(ec * 4 + f) * 2 + 1];
+  }
+  const unsigned setupAll = (lts.setup != nullptr && active) ? lts.setup[e] : 0u;
+  // the neighbour buffer of a face, gathered into registers (unconditional,
+  // clamped addresses; a face that takes derivatives or is not regular
+  // discards the values); FP32: one face ahead, so that the loads are in
+  // flight during the previous face's products (FP64: at the face, the
+  // registers of a prefetch make the kernel spill)
+  constexpr bool kAhead = sizeof(real) == 4;
+  constexpr int GI = (S::QS + TPE - 1) / TPE;
+  real gv[GI];
+  // entry f of a per-face register array (a select chain, not an indexed access)
+  auto pick = [](const auto (&v)[4], int f) { return f == 0 ? v[0] : (f == 1 ? v[1] : (f == 2 ? v[2] : v[3])); };
+  auto gather = [&](int64_t nb) {
+#pragma unroll
+    for (int c = 0; c < GI; ++c) {
+      const int idx = min(t + c * TPE, S::QS - 1);
+      gv[c] = integ[nb * lay.qStride + (idx / B) * lay.ldq + idx % B];
+    }
+  };
+  if constexpr (kAhead) gather(nbs[0]);
+
 #pragma unroll 1
   for (int f = 0; f < 4; ++f) {
-    const bool regular = active && (faceTypes == nullptr || faceTypes[e * 4 + f] == SDG_FACE_REGULAR);
-    const int64_t nb = regular ? neighbors[e * 4 + f] : 0;
+    const bool regular = pick(regs, f);
+    const int64_t nb = pick(nbs, f);
+    const uint8_t side = f == 0 ? rels[0][0] : (f == 1 ? rels[1][0] : (f == 2 ? rels[2][0] : rels[3][0]));
+    const uint8_t orient = f == 0 ? rels[0][1] : (f == 1 ? rels[1][1] : (f == 2 ? rels[2][1] : rels[3][1]));
     // clamp so malformed relations can never index outside the matrix table
-    const int j = regular ? min(static_cast<int>(faceRel[(e * 4 + f) * 2 + 0]), 3) : 0;
-    const int h = regular ? min(static_cast<int>(faceRel[(e * 4 + f) * 2 + 1]), 2) : 0;
+    const int j = regular ? min(static_cast<int>(side), 3) : 0;
+    const int h = regular ? min(static_cast<int>(orient), 2) : 0;
     // the neighbour's time integral: its buffer, or its derivatives integrated
     // over this step (TimeCommon.cpp:40-87): GTS relation -> time coefficients,
     // else the sub-interval coefficients
-    const unsigned setup = (lts.setup != nullptr && regular) ? lts.setup[e] : 0u;
+    const unsigned setup = regular ? setupAll : 0u;
     const bool fromDerivatives = (setup >> f) & 1u;
     const Coeffs<O, real>& cf = ((setup >> (4 + f)) & 1u) ? lts.gts : lts.sub;
+    if constexpr (!kAhead) gather(nb);
     if (laneOk && !fromDerivatives) {
-      // the neighbour's buffer (zeros off regular faces) in chunks of six
-      // loads per lane, each chunk issued before its first value is stored
-      // (clamped addresses, in bounds; nb = 0 off regular faces)
-      constexpr int GI = (S::QS + TPE - 1) / TPE;
 #pragma unroll
-      for (int c = 0; c < GI; c += 6) {
-        real gv[6];
-#pragma unroll
-        for (int u = 0; u < 6; ++u) {
-          const int idx = min(t + (c + u) * TPE, S::QS - 1);
-          gv[u] = integ[nb * lay.qStride + (idx / B) * lay.ldq + idx % B];
-        }
-#pragma unroll
-        for (int u = 0; u < 6; ++u) {
-          const int idx = t + (c + u) * TPE;
-          if (c + u < GI && idx < S::QS) xa[(idx % B) * 9 + idx / B] = regular ? gv[u] : real(0);
-        }
+      for (int c = 0; c < GI; ++c) {
+        const int idx = t + c * TPE;
+        if (idx < S::QS) xa[(idx % B) * 9 + idx / B] = regular ? gv[c] : real(0);
       }
     } else if (laneOk) {
       // the neighbour's derivatives integrated over the interval: the O
@@ -652,6 +675,8 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
         xa[l * 9 + p] = v;
       }
     }
+    if constexpr (kAhead)
+      if (f + 1 < 4) gather(pick(nbs, f + 1));  // in flight during this face's products
     __syncthreads();
     {
       real u[1][R][9];
