@@ -197,8 +197,9 @@ int sdg_compute_batched_ader_local(sdg_engine* engine, const double* coeffs, dou
 
 /* ---------------------------------------------------------------------- */
 /* Local time stepping (rate 2).  Every engine: the matrix-core engines    */
-/* (orders 3-7) run the local pass on the matrix cores and the neighbour    */
-/* pass on the FP32 VALU kernel.                                            */
+/* (orders 3-7) run the local pass on the matrix cores; the neighbour pass  */
+/* runs on the VALU kernel of the engine's real type (FP32 for the split   */
+/* forms, FP64 for the FP64 engine).                                        */
 /* ---------------------------------------------------------------------- */
 /* deriveLtsSetups (src/Initializer/InitProcedure/Internal/LtsSetup.cpp:97-330,
  * getLtsSetup + normalizeLtsSetup), host: LTS bits of every cell from the
