@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "cell_local.h"
@@ -95,128 +96,228 @@ __device__ inline int derivativeOffset(int i, const CellLayout& lay) {
   return at;
 }
 
+// compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1
+template <int N, int I = 0, typename F>
+__device__ inline void staticFor(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    staticFor<N, I + 1>(f);
+  }
+}
+
+// acc[i] += (lane J of row[i] within its row of 16 lanes) * x, i = 0..8:
+// nine FMAs whose first factor is broadcast across each 16-lane row (DPP
+// row_newbcast).  A row register read from LDS once (lane c of each row holds
+// element c of a 16-element run) thereby serves as sixteen wave-uniform
+// operands, where a broadcast read from LDS would return every value to all
+// 64 lanes.  The leading s_nop 1 gives the two wait states a DPP read needs
+// after a VALU write of its source register.
+template <int J, typename real>
+__device__ inline void fmacRowBcast9(real (&acc)[9], const real (&row)[9], real x) {
+  static_assert(J >= 0 && J < 16, "row_newbcast selects a lane of a 16-lane row");
+  if constexpr (sizeof(real) == 8) {
+    asm("s_nop 1\n\t"
+        "v_fmac_f64_dpp %0, %9, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %1, %10, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %2, %11, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %3, %12, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %4, %13, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %5, %14, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %6, %15, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %7, %16, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %8, %17, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]), "+v"(acc[6]),
+          "+v"(acc[7]), "+v"(acc[8])
+        : "v"(row[0]), "v"(row[1]), "v"(row[2]), "v"(row[3]), "v"(row[4]), "v"(row[5]), "v"(row[6]),
+          "v"(row[7]), "v"(row[8]), "v"(x), "i"(J));
+  } else {
+    asm("s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %9, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %1, %10, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %2, %11, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %3, %12, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %4, %13, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %5, %14, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %6, %15, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %7, %16, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %8, %17, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]), "+v"(acc[6]),
+          "+v"(acc[7]), "+v"(acc[8])
+        : "v"(row[0]), "v"(row[1]), "v"(row[2]), "v"(row[3]), "v"(row[4]), "v"(row[5]), "v"(row[6]),
+          "v"(row[7]), "v"(row[8]), "v"(x), "i"(J));
+  }
+}
+
 // Space-time interpolation, batched by the 16 variants of V3mTo2n(i, h)
 // (h = 0: plus side, h = 1..3: minus side with that orientation), as the
 // reference batches per (side, faceRelation): workgroup (chunk, variant)
 // stages its matrix in LDS once and serves every side of its variant among
-// the chunk's kDrChunk faces, one side per wave.  Per side the derivatives
-// are interpolated first, w_d[k][q] = sum_l V[k][l] dQ_d[q][l] (lane k =
-// Stroud point), then the time points Q_o = sum_d c[o][d] w_d are rotated,
+// the chunk's kDrChunk faces, listed in face order and dealt round-robin to
+// its waves (one side per wave at a time).  Per side the derivatives are
+// interpolated first, w_d[k][q] = sum_l V[k][l] dQ_d[q][l] (lane k = Stroud
+// point), then the time points Q_o = sum_d c[o][d] w_d are rotated,
 // out[f][o][p][k] = sum_q Q_o[k][q] TinvT(q, p) (FP64, the friction law's
 // q_interpolated layout; points k in [(O+1)^2, P) are written as zero).
+// The gather of a wave's next side is issued before the arithmetic of its
+// current one, so that the loads are in flight while it computes.
 template <typename real, int O>
 __global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, const real* __restrict__ V,
                                                                 const real* __restrict__ deriv, CellLayout lay,
                                                                 sdg_dr_layer L, StepCoeffs<O> cf,
                                                                 double* __restrict__ outPlus,
                                                                 double* __restrict__ outMinus) {
+  static_assert(kDrChunk == 64 * kDrWaves, "one face of the chunk per thread");
   constexpr int B = numBasis3(O), NP = (O + 1) * (O + 1), LDV = B + 1;
   constexpr int DS = [] {
     int t = 0;
     for (int d = 0; d < O; ++d) t += 9 * numBasis3(O - d);
     return t;
   }();
+  constexpr int DI = (DS + 63) / 64;  // derivative values per lane
   __shared__ real sV[NP * LDV];
-  __shared__ real sD[kDrWaves][DS];
-  __shared__ real sT[kDrWaves][81];
-  __shared__ int sElem[kDrChunk];  // the chunk's element of this variant's side, -1: other variant
+  // padded by 16: a row read starting at the last run of a block reaches past it
+  __shared__ real sD[kDrWaves][DS + 16];
+  __shared__ real sT[kDrWaves][96];
+  __shared__ int sFace[kDrChunk];  // the chunk's sides of this variant: chunk-local face index
+  __shared__ int sElem[kDrChunk];  // and the side's element
+  __shared__ int sCount[kDrWaves];
   const int variant = blockIdx.y, vi = variant >> 2, vh = variant & 3;
   const int side = vh == 0 ? 0 : 1;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const real* Vv = V + static_cast<int64_t>(vi * 4 + vh) * NP * B;
   const int64_t f0 = static_cast<int64_t>(blockIdx.x) * kDrChunk;
-  if (threadIdx.x < kDrChunk) {
+  bool match = false;
+  int elem = 0;
+  {
     const int64_t f = f0 + threadIdx.x;
-    int elem = -1;
     if (f < nf) {
       const int faceSide = min(static_cast<int>(L.face_info[f * 3 + side]), 3);
       // faceRelation = orientation + 1 lies in [1, 3]; clamped so that a malformed
       // entry still names a minus-side variant (0 would match no workgroup)
       const int h = side == 0 ? 0 : min(max(static_cast<int>(L.face_info[f * 3 + 2]), 1), 3);
-      if (faceSide == vi && h == vh) elem = side == 0 ? L.plus_element[f] : L.minus_element[f];
+      match = faceSide == vi && h == vh;
+      elem = side == 0 ? L.plus_element[f] : L.minus_element[f];
     }
-    sElem[threadIdx.x] = elem;
   }
+  const unsigned long long mask = __ballot(match);
+  if (lane == 0) sCount[wave] = __popcll(mask);
   for (int i = threadIdx.x; i < NP * B; i += blockDim.x) sV[(i / B) * LDV + i % B] = Vv[i];
+  __syncthreads();
+  int base = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kDrWaves; ++w) {
+    const int c = sCount[w];
+    base += w < wave ? c : 0;
+    total += c;
+  }
+  total = __builtin_amdgcn_readfirstlane(total);
+  if (match) {
+    const int at = base + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                              static_cast<unsigned>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u)));
+    sFace[at] = threadIdx.x;
+    sElem[at] = elem;
+  }
   __syncthreads();
   const int P = L.points;
   double* out = side == 0 ? outPlus : outMinus;
-  for (int j = wave; j < kDrChunk; j += kDrWaves) {
-    const int64_t f = f0 + j;
-    const int elem = sElem[j];
-    if (elem < 0) continue;
-    const int64_t e = elem;
-    waveSync();  // the previous side's reads of sD/sT are done
-    // the element's derivatives in chunks of six loads per lane, each chunk
-    // issued before its first value is stored (clamped addresses, in bounds)
-    constexpr int DI = (DS + 63) / 64;
-    // the gather offsets are recomputed per face (an opaque copy of the lane
-    // index): hoisted out of the face loop they are spilled, and every load
-    // then waits on the reload of its address
+  real pre[DI];  // the next side's derivatives, in flight during the current side
+  double preT[2];
+  auto fetch = [&](int j) {
+    const int64_t e = __builtin_amdgcn_readfirstlane(sElem[j]);
+    const int64_t f = f0 + __builtin_amdgcn_readfirstlane(sFace[j]);
+    const real* src = deriv + e * lay.dsize;
+    // an opaque copy of the lane index: the gather offsets are recomputed per
+    // side (hoisted out of the side loop they are spilled, and every load then
+    // waits on the reload of its address)
     int ln = lane;
     asm volatile("" : "+v"(ln));
+    if (lay.align == 1) {
 #pragma unroll
-    for (int c = 0; c < DI; c += 6) {
-      real tmp[6];
+      for (int c = 0; c < DI; ++c) pre[c] = src[min(ln + 64 * c, DS - 1)];  // clamped: in bounds
+    } else {
 #pragma unroll
-      for (int u = 0; u < 6; ++u) {
-        const int i = min(ln + 64 * (c + u), DS - 1);
-        tmp[u] = deriv[e * lay.dsize + derivativeOffset<O>(i, lay)];
-      }
-#pragma unroll
-      for (int u = 0; u < 6; ++u) {
-        const int i = lane + 64 * (c + u);
-        if (c + u < DI && i < DS) sD[wave][i] = tmp[u];
-      }
+      for (int c = 0; c < DI; ++c) pre[c] = src[derivativeOffset<O>(min(ln + 64 * c, DS - 1), lay)];
     }
-    for (int i = lane; i < 81; i += 64) sT[wave][i] = static_cast<real>(L.tinv_t[f * 81 + i]);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) preT[c] = L.tinv_t[f * 81 + min(ln + 64 * c, 80)];
+  };
+  if (wave < total) fetch(wave);
+  for (int j = wave; j < total; j += kDrWaves) {
+    const int64_t f = f0 + __builtin_amdgcn_readfirstlane(sFace[j]);
+    waveSync();  // the previous side's reads of sD/sT are done
+#pragma unroll
+    for (int c = 0; c < DI; ++c) {
+      const int i = lane + 64 * c;
+      if (c + 1 < DI || i < DS) sD[wave][i] = pre[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int i = lane + 64 * c;
+      if (i < 81) sT[wave][i] = static_cast<real>(preT[c]);
+    }
     waveSync();
-    const int k = lane;
+    if (j + kDrWaves < total) fetch(j + kDrWaves);
+    // lanes k >= NP compute with the last point's row and store zero or nothing
+    const int k = min(lane, NP - 1);
+    const int c16 = lane & 15;
     real w[O][9];
 #pragma unroll
     for (int d = 0; d < O; ++d)
 #pragma unroll
       for (int q = 0; q < 9; ++q) w[d][q] = real(0);
-    if (k < NP) {
-      // derivative order outer, basis index inner: branch-free bodies with
-      // compile-time trip counts that the compiler can unroll and pipeline
-      int off = 0;
+    // w_d[q] += V[k][l] dQ_d[q][l] over runs of 16 basis functions: row q of a
+    // run holds dQ_d[q][l0 + c16], and step j broadcasts its lane j
+    staticFor<O>([&](auto dI) {
+      constexpr int d = decltype(dI)::value;
+      constexpr int m = numBasis3(O - d);
+      constexpr int off = [] {
+        int t = 0;
+        for (int e = 0; e < d; ++e) t += 9 * numBasis3(O - e);
+        return t;
+      }();
+      staticFor<(m + 15) / 16>([&](auto rI) {
+        constexpr int l0 = 16 * decltype(rI)::value;
+        constexpr int nl = m - l0 < 16 ? m - l0 : 16;
+        // the run's rows and basis values are all read before its first FMA
+        real row[9], xv[nl];
 #pragma unroll
-      for (int d = 0; d < O; ++d) {
-        const int m = numBasis3(O - d);
-#pragma unroll 4
-        for (int l = 0; l < m; ++l) {
-          const real v = sV[k * LDV + l];
+        for (int q = 0; q < 9; ++q) row[q] = sD[wave][off + q * m + l0 + c16];
 #pragma unroll
-          for (int q = 0; q < 9; ++q) w[d][q] += v * sD[wave][off + q * m + l];
-        }
-        off += 9 * m;
-      }
-    }
-    // rotation commutes with the time combination: rotate each w_d in place
+        for (int jj = 0; jj < nl; ++jj) xv[jj] = sV[k * LDV + l0 + jj];
+        staticFor<nl>([&](auto jI) {
+          constexpr int jj = decltype(jI)::value;
+          fmacRowBcast9<jj>(w[d], row, xv[jj]);
+        });
+      });
+    });
+    // rows of the rotation: rowT[p] holds TinvT(c16, p) (lanes c16 < 9 are read)
+    real rowT[9];
 #pragma unroll
-    for (int d = 0; d < O; ++d) {
-      real tmp[9];
+    for (int p = 0; p < 9; ++p) rowT[p] = sT[wave][9 * p + c16];
+    // per time point: Q_o = sum_d c[o][d] w_d, then rotated,
+    // out[p] = sum_q Q_o[q] TinvT(q, p)
 #pragma unroll
-      for (int q = 0; q < 9; ++q) tmp[q] = w[d][q];
+    for (int o = 0; o < O; ++o) {
+      real qo[9];
 #pragma unroll
-      for (int p = 0; p < 9; ++p) {
+      for (int q = 0; q < 9; ++q) {
         real acc = real(0);
 #pragma unroll
-        for (int q = 0; q < 9; ++q) acc += tmp[q] * sT[wave][q + 9 * p];
-        w[d][p] = acc;
+        for (int d = 0; d < O; ++d) acc += static_cast<real>(cf.c[o][d]) * w[d][q];
+        qo[q] = acc;
       }
+      real acc[9];
+#pragma unroll
+      for (int p = 0; p < 9; ++p) acc[p] = real(0);
+      staticFor<9>([&](auto qI) {
+        constexpr int q = decltype(qI)::value;
+        fmacRowBcast9<q>(acc, rowT, qo[q]);
+      });
+#pragma unroll
+      for (int p = 0; p < 9; ++p)
+        if (lane < P) out[((f * O + o) * 9 + p) * P + lane] = lane < NP ? static_cast<double>(acc[p]) : 0.0;
     }
-#pragma unroll
-    for (int o = 0; o < O; ++o)
-#pragma unroll
-      for (int p = 0; p < 9; ++p) {
-        real acc = real(0);
-#pragma unroll
-        for (int d = 0; d < O; ++d) acc += static_cast<real>(cf.c[o][d]) * w[d][p];
-        if (k < P) out[((f * O + o) * 9 + p) * P + k] = k < NP ? static_cast<double>(acc) : 0.0;
-      }
   }
 }
 
