@@ -52,7 +52,8 @@ bool drLayerOk(const sdg_dr_layer* L, int order, const char* who) {
 }
 
 // face matrices of the engine's order on the device (real type of the engine):
-// V3mTo2n [4][4][n][B], V3mTo2nTWDivM [4][4][B][n]
+// V3mTo2n [4][4][n][B], then V3mTo2nTWDivM transposed, [4][4][n][B] (the
+// nodal flux reads it over the lanes' basis functions)
 template <typename real>
 int uploadDrMatrices(EngineState& s) {
   const int O = s.order, B = numBasis3(O), n = stroudPoints(O);
@@ -60,7 +61,11 @@ int uploadDrMatrices(EngineState& s) {
   if (sdg_dr_face_matrices(O, v.data(), vt.data()) != SDG_OK) return SDG_ERR_INVALID;
   std::vector<real> both(v.size() + vt.size());
   for (size_t i = 0; i < v.size(); ++i) both[i] = static_cast<real>(v[i]);
-  for (size_t i = 0; i < vt.size(); ++i) both[v.size() + i] = static_cast<real>(vt[i]);
+  for (int fh = 0; fh < 16; ++fh)
+    for (int k = 0; k < B; ++k)
+      for (int l = 0; l < n; ++l)
+        both[v.size() + (static_cast<size_t>(fh) * n + l) * B + k] =
+            static_cast<real>(vt[(static_cast<size_t>(fh) * B + k) * n + l]);
   SDG_HIP_CHECK(hipMalloc(&s.drMatrices, both.size() * sizeof(real)));
   SDG_HIP_CHECK(hipMemcpy(s.drMatrices, both.data(), both.size() * sizeof(real), hipMemcpyHostToDevice));
   return SDG_OK;
@@ -71,8 +76,9 @@ int ensureDrMatrices(EngineState& s) {
   return s.arithmetic == SDG_FP64 ? uploadDrMatrices<double>(s) : uploadDrMatrices<float>(s);
 }
 
-constexpr int kDrChunk = 256;  // faces scanned by one interpolation workgroup (one per thread)
-constexpr int kDrWaves = 4;   // sides in flight per workgroup (one per wave)
+constexpr int kDrWaves = 4;   // sides in flight per interpolation workgroup (one per wave)
+constexpr int kDrScan = 1;    // faces of the chunk per thread
+constexpr int kDrChunk = 64 * kDrWaves * kDrScan;  // faces scanned by one interpolation workgroup
 
 __device__ inline void waveSync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -94,6 +100,15 @@ __device__ inline int derivativeOffset(int i, const CellLayout& lay) {
     off += 9 * lay.rows(m);
   }
   return at;
+}
+
+// first column (d, l = 0) of derivative order d among the columns of all
+// orders, sum_{e < d} B(O - e); the packed block of order d starts at 9 times it
+template <int O>
+__host__ __device__ constexpr int columnOffset(int d) {
+  int t = 0;
+  for (int e = 0; e < d; ++e) t += numBasis3(O - e);
+  return t;
 }
 
 // compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1
@@ -154,9 +169,11 @@ __device__ inline void fmacRowBcast9(real (&acc)[9], const real (&row)[9], real 
 // stages its matrix in LDS once and serves every side of its variant among
 // the chunk's kDrChunk faces, listed in face order and dealt round-robin to
 // its waves (one side per wave at a time).  Per side the derivatives are
-// interpolated first, w_d[k][q] = sum_l V[k][l] dQ_d[q][l] (lane k = Stroud
-// point), then the time points Q_o = sum_d c[o][d] w_d are rotated,
-// out[f][o][p][k] = sum_q Q_o[k][q] TinvT(q, p) (FP64, the friction law's
+// rotated to the face frame first, dQ'_d[p][l] = sum_q dQ_d[q][l] TinvT(q, p)
+// (the rotation commutes with the interpolation; columns (d, l) spread over
+// the lanes), then interpolated, w_d[k][p] = sum_l V[k][l] dQ'_d[p][l] (lane
+// k = Stroud point), and combined at the time points,
+// out[f][o][p][k] = sum_d c[o][d] w_d[k][p] (FP64, the friction law's
 // q_interpolated layout; points k in [(O+1)^2, P) are written as zero).
 // The gather of a wave's next side is issued before the arithmetic of its
 // current one, so that the loads are in flight while it computes.
@@ -166,7 +183,6 @@ __global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, c
                                                                 sdg_dr_layer L, StepCoeffs<O> cf,
                                                                 double* __restrict__ outPlus,
                                                                 double* __restrict__ outMinus) {
-  static_assert(kDrChunk == 64 * kDrWaves, "one face of the chunk per thread");
   constexpr int B = numBasis3(O), NP = (O + 1) * (O + 1), LDV = B + 1;
   constexpr int DS = [] {
     int t = 0;
@@ -174,49 +190,74 @@ __global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, c
     return t;
   }();
   constexpr int DI = (DS + 63) / 64;  // derivative values per lane
+  constexpr int NCOL = DS / 9;        // columns (d, l) of the derivatives
+  constexpr int NT = 64 * kDrWaves;   // threads
+  constexpr int VI = (NP * B + NT - 1) / NT;  // matrix entries staged per thread
   __shared__ real sV[NP * LDV];
   // padded by 16: a row read starting at the last run of a block reaches past it
   __shared__ real sD[kDrWaves][DS + 16];
   __shared__ real sT[kDrWaves][96];
   __shared__ int sFace[kDrChunk];  // the chunk's sides of this variant: chunk-local face index
   __shared__ int sElem[kDrChunk];  // and the side's element
-  __shared__ int sCount[kDrWaves];
+  __shared__ int sCount[kDrScan][kDrWaves];
   const int variant = blockIdx.y, vi = variant >> 2, vh = variant & 3;
   const int side = vh == 0 ? 0 : 1;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const real* Vv = V + static_cast<int64_t>(vi * 4 + vh) * NP * B;
   const int64_t f0 = static_cast<int64_t>(blockIdx.x) * kDrChunk;
-  bool match = false;
-  int elem = 0;
-  {
-    const int64_t f = f0 + threadIdx.x;
-    if (f < nf) {
-      const int faceSide = min(static_cast<int>(L.face_info[f * 3 + side]), 3);
-      // faceRelation = orientation + 1 lies in [1, 3]; clamped so that a malformed
-      // entry still names a minus-side variant (0 would match no workgroup)
-      const int h = side == 0 ? 0 : min(max(static_cast<int>(L.face_info[f * 3 + 2]), 1), 3);
-      match = faceSide == vi && h == vh;
-      elem = side == 0 ? L.plus_element[f] : L.minus_element[f];
-    }
-  }
-  const unsigned long long mask = __ballot(match);
-  if (lane == 0) sCount[wave] = __popcll(mask);
-  for (int i = threadIdx.x; i < NP * B; i += blockDim.x) sV[(i / B) * LDV + i % B] = Vv[i];
-  __syncthreads();
-  int base = 0, total = 0;
+  // the start-up loads (the variant's matrix, the chunk's face data), all
+  // issued before the first is used
+  real vv[VI];
 #pragma unroll
-  for (int w = 0; w < kDrWaves; ++w) {
-    const int c = sCount[w];
-    base += w < wave ? c : 0;
-    total += c;
+  for (int u = 0; u < VI; ++u) vv[u] = Vv[min(static_cast<int>(threadIdx.x) + NT * u, NP * B - 1)];
+  bool match[kDrScan];
+  int elem[kDrScan];
+#pragma unroll
+  for (int s = 0; s < kDrScan; ++s) {
+    const int64_t f = min(f0 + threadIdx.x + NT * s, nf - 1);  // nf >= 1 here
+    const int faceSide = min(static_cast<int>(L.face_info[f * 3 + side]), 3);
+    // faceRelation = orientation + 1 lies in [1, 3]; clamped so that a malformed
+    // entry still names a minus-side variant (0 would match no workgroup)
+    const int h = side == 0 ? 0 : min(max(static_cast<int>(L.face_info[f * 3 + 2]), 1), 3);
+    const int el = side == 0 ? L.plus_element[f] : L.minus_element[f];
+    match[s] = f0 + threadIdx.x + NT * s < nf && faceSide == vi && h == vh;
+    elem[s] = el;
   }
+  unsigned long long mask[kDrScan];
+#pragma unroll
+  for (int s = 0; s < kDrScan; ++s) {
+    mask[s] = __ballot(match[s]);
+    if (lane == 0) sCount[s][wave] = __popcll(mask[s]);
+  }
+#pragma unroll
+  for (int u = 0; u < VI; ++u) {
+    const int i = threadIdx.x + NT * u;
+    if (u + 1 < VI || i < NP * B) sV[(i / B) * LDV + i % B] = vv[u];
+  }
+  __syncthreads();
+  // the chunk's sides of this variant in face order: scan s, then wave
+  int total = 0;
+#pragma unroll
+  for (int s = 0; s < kDrScan; ++s)
+#pragma unroll
+    for (int w = 0; w < kDrWaves; ++w) total += sCount[s][w];
   total = __builtin_amdgcn_readfirstlane(total);
-  if (match) {
-    const int at = base + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
-                              static_cast<unsigned>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u)));
-    sFace[at] = threadIdx.x;
-    sElem[at] = elem;
+#pragma unroll
+  for (int s = 0; s < kDrScan; ++s) {
+    int base = 0;
+#pragma unroll
+    for (int s1 = 0; s1 < kDrScan; ++s1)
+#pragma unroll
+      for (int w = 0; w < kDrWaves; ++w)
+        base += (s1 < s || (s1 == s && w < wave)) ? sCount[s1][w] : 0;
+    if (match[s]) {
+      const int at = base + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                                static_cast<unsigned>(mask[s] >> 32),
+                                __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask[s]), 0u)));
+      sFace[at] = threadIdx.x + NT * s;
+      sElem[at] = elem[s];
+    }
   }
   __syncthreads();
   const int P = L.points;
@@ -258,9 +299,49 @@ __global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, c
     }
     waveSync();
     if (j + kDrWaves < total) fetch(j + kDrWaves);
-    // lanes k >= NP compute with the last point's row and store zero or nothing
-    const int k = min(lane, NP - 1);
     const int c16 = lane & 15;
+    // rotation to the face frame first (it commutes with the interpolation):
+    // dQ_d[p][l] <- sum_q dQ_d[q][l] TinvT(q, p) in place, one column (d, l)
+    // per lane and pass; rowT[p] holds TinvT(c16, p)
+    {
+      real rowT[9];
+#pragma unroll
+      for (int p = 0; p < 9; ++p) rowT[p] = sT[wave][9 * p + c16];
+      // one column at a time (two in flight would hold 36 more registers)
+#pragma unroll 1
+      for (int cc = 0; cc < (NCOL + 63) / 64; ++cc) {
+        const int col = min(lane + 64 * cc, NCOL - 1);  // clamped: read, not written
+        int at = 0, stride = 1;
+        staticFor<O>([&](auto dI) {
+          constexpr int d = decltype(dI)::value;
+          constexpr int m = numBasis3(O - d);
+          constexpr int c0 = columnOffset<O>(d);
+          if (col >= c0 && col < c0 + m) {
+            at = 9 * c0 + (col - c0);  // the packed block of order d starts at 9 * c0
+            stride = m;
+          }
+        });
+        real x[9], y[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+          x[q] = sD[wave][at + q * stride];
+          y[q] = real(0);
+        }
+        staticFor<9>([&](auto qI) {
+          constexpr int q = decltype(qI)::value;
+          fmacRowBcast9<q>(y, rowT, x[q]);
+        });
+        if (lane + 64 * cc < NCOL)
+#pragma unroll
+          for (int p = 0; p < 9; ++p) sD[wave][at + p * stride] = y[p];
+      }
+    }
+    waveSync();
+    // lanes k >= NP compute with the last point's row and store zero or nothing.
+    // The row offset is an opaque copy per side: the basis values are loop
+    // invariant, and hoisted out of the side loop they are spilled
+    int vk = min(lane, NP - 1) * LDV;
+    asm volatile("" : "+v"(vk));
     real w[O][9];
 #pragma unroll
     for (int d = 0; d < O; ++d)
@@ -271,11 +352,7 @@ __global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, c
     staticFor<O>([&](auto dI) {
       constexpr int d = decltype(dI)::value;
       constexpr int m = numBasis3(O - d);
-      constexpr int off = [] {
-        int t = 0;
-        for (int e = 0; e < d; ++e) t += 9 * numBasis3(O - e);
-        return t;
-      }();
+      constexpr int off = 9 * columnOffset<O>(d);
       staticFor<(m + 15) / 16>([&](auto rI) {
         constexpr int l0 = 16 * decltype(rI)::value;
         constexpr int nl = m - l0 < 16 ? m - l0 : 16;
@@ -284,39 +361,35 @@ __global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, c
 #pragma unroll
         for (int q = 0; q < 9; ++q) row[q] = sD[wave][off + q * m + l0 + c16];
 #pragma unroll
-        for (int jj = 0; jj < nl; ++jj) xv[jj] = sV[k * LDV + l0 + jj];
+        for (int jj = 0; jj < nl; ++jj) xv[jj] = sV[vk + l0 + jj];
         staticFor<nl>([&](auto jI) {
           constexpr int jj = decltype(jI)::value;
           fmacRowBcast9<jj>(w[d], row, xv[jj]);
         });
       });
     });
-    // rows of the rotation: rowT[p] holds TinvT(c16, p) (lanes c16 < 9 are read)
-    real rowT[9];
-#pragma unroll
-    for (int p = 0; p < 9; ++p) rowT[p] = sT[wave][9 * p + c16];
-    // per time point: Q_o = sum_d c[o][d] w_d, then rotated,
-    // out[p] = sum_q Q_o[q] TinvT(q, p)
+    // per time point: out[o][p] = sum_d c[o][d] w_d[p] (the rotated w).  The
+    // lane index of the store offsets is an opaque copy per side: hoisted out
+    // of the side loop, the 54 offsets are spilled
+    double* const dst = out + f * (O * 9) * P;
+    int sl = lane;
+    asm volatile("" : "+v"(sl));
 #pragma unroll
     for (int o = 0; o < O; ++o) {
-      real qo[9];
-#pragma unroll
-      for (int q = 0; q < 9; ++q) {
-        real acc = real(0);
-#pragma unroll
-        for (int d = 0; d < O; ++d) acc += static_cast<real>(cf.c[o][d]) * w[d][q];
-        qo[q] = acc;
-      }
       real acc[9];
 #pragma unroll
-      for (int p = 0; p < 9; ++p) acc[p] = real(0);
-      staticFor<9>([&](auto qI) {
-        constexpr int q = decltype(qI)::value;
-        fmacRowBcast9<q>(acc, rowT, qo[q]);
-      });
+      for (int p = 0; p < 9; ++p) {
+        real a = real(0);
+#pragma unroll
+        for (int d = 0; d < O; ++d) a += static_cast<real>(cf.c[o][d]) * w[d][p];
+        acc[p] = a;
+      }
 #pragma unroll
       for (int p = 0; p < 9; ++p)
-        if (lane < P) out[((f * O + o) * 9 + p) * P + lane] = lane < NP ? static_cast<double>(acc[p]) : 0.0;
+        if (lane < P) dst[(o * 9 + p) * P + sl] = lane < NP ? static_cast<double>(acc[p]) : 0.0;
+      // one time point at a time: interleaved, the time points' sums and
+      // store addresses outgrow the registers
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
@@ -324,21 +397,28 @@ __global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, c
 // One workgroup per cell: for each DR face of the cell,
 // Q[p][k] += sum_l VTW[k][l] sum_q G[q][l] F(q, p) with the imposed state G
 // and flux solver F of the cell's side (plus: h = 0; minus: h = faceRelation).
+// Both contractions take their face-uniform factor (F, then GF) from 16-lane
+// rows broadcast with DPP row_newbcast; VTW is read as VTWt[l][k].
 template <typename real, int O>
-__global__ __launch_bounds__(kDrThreads) void kDrNodalFlux(int64_t n, const real* __restrict__ VTW,
+__global__ __launch_bounds__(kDrThreads) void kDrNodalFlux(int64_t n, const real* __restrict__ VTWt,
                                                            const int32_t* __restrict__ drMap, sdg_dr_layer L,
                                                            const double* __restrict__ impPlus,
                                                            const double* __restrict__ impMinus,
                                                            real* __restrict__ dofs, CellLayout lay) {
+  static_assert(kDrThreads == 64, "one wave per cell");
   constexpr int B = numBasis3(O), NP = (O + 1) * (O + 1);
-  __shared__ real sGF[64 * 9];
-  __shared__ real sF[81];
+  constexpr int R = (B + kDrThreads - 1) / kDrThreads;  // basis functions per lane
+  // GF as [p][l]; both arrays padded by 16 for the row reads of a last run
+  __shared__ real sGF[9 * 64 + 16];
+  __shared__ real sF[81 + 16];
   const int64_t e = blockIdx.x;
   if (e >= n) return;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, c16 = tid & 15;
   const int P = L.points;
-  real acc[(B + kDrThreads - 1) / kDrThreads][9];
-  for (int r = 0; r < (B + kDrThreads - 1) / kDrThreads; ++r)
+  real acc[R][9];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
     for (int p = 0; p < 9; ++p) acc[r][p] = real(0);
   bool any = false;
   for (int face = 0; face < 4; ++face) {
@@ -356,38 +436,63 @@ __global__ __launch_bounds__(kDrThreads) void kDrNodalFlux(int64_t n, const real
     const int h = plus ? 0 : min(max(relation, 1), 3);  // minus side: faceRelation in [1, 3]
     const double* G = (plus ? impPlus : impMinus) + static_cast<int64_t>(m) * 9 * P;
     const double* F = (plus ? L.flux_solver_plus : L.flux_solver_minus) + static_cast<int64_t>(m) * 81;
+    // the face's loads, all issued before the first wait: G at the lane's
+    // point (clamped: lanes l >= NP compute a copy of the last point), F
+    const int l = min(tid, NP - 1);
+    real g[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) g[q] = static_cast<real>(G[q * P + l]);
+    real fv[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) fv[c] = static_cast<real>(F[min(tid + 64 * c, 80)]);
+    __syncthreads();  // the previous face's reads of sF and sGF are done
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      if (tid + 64 * c < 81) sF[tid + 64 * c] = fv[c];
     __syncthreads();
-    for (int i = tid; i < 81; i += kDrThreads) sF[i] = static_cast<real>(F[i]);
-    __syncthreads();
-    // GF[l][p] = sum_q G[q][l] F(q, p)
-    if (tid < NP) {
+    // GF[l][p] = sum_q G[q][l] F(q, p); rowF[p] holds F(c16, p)
+    {
+      real rowF[9], gf[9];
 #pragma unroll
       for (int p = 0; p < 9; ++p) {
-        real s = real(0);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) s += static_cast<real>(G[q * P + tid]) * sF[q + 9 * p];
-        sGF[tid * 9 + p] = s;
+        rowF[p] = sF[9 * p + c16];
+        gf[p] = real(0);
       }
+      staticFor<9>([&](auto qI) {
+        constexpr int q = decltype(qI)::value;
+        fmacRowBcast9<q>(gf, rowF, g[q]);
+      });
+#pragma unroll
+      for (int p = 0; p < 9; ++p)
+        if (tid < NP) sGF[p * 64 + tid] = gf[p];
     }
     __syncthreads();
-    const real* W = VTW + ((static_cast<int64_t>(face) * 4 + h) * B) * NP;
-    for (int r = 0; r < (B + kDrThreads - 1) / kDrThreads; ++r) {
-      const int k = tid + r * kDrThreads;
-      if (k < B)
-        // unrolled so that several loads of W are in flight at once (rolled,
-        // each iteration waited on its load)
-#pragma unroll 7
-        for (int l = 0; l < NP; ++l) {
-          const real w = W[k * NP + l];
+    // Q[p][k] += sum_l VTW[k][l] GF[l][p] over runs of 16 points: row p of a
+    // run holds GF[l0 + c16][p], and step j broadcasts its lane j
+    const real* W = VTWt + static_cast<int64_t>(face * 4 + h) * NP * B;
 #pragma unroll
-          for (int p = 0; p < 9; ++p) acc[r][p] += w * sGF[l * 9 + p];
-        }
+    for (int r = 0; r < R; ++r) {
+      const int k = min(tid + kDrThreads * r, B - 1);
+      staticFor<(NP + 15) / 16>([&](auto rI) {
+        constexpr int l0 = 16 * decltype(rI)::value;
+        constexpr int nl = NP - l0 < 16 ? NP - l0 : 16;
+        real row[9], xv[nl];
+#pragma unroll
+        for (int jj = 0; jj < nl; ++jj) xv[jj] = W[(l0 + jj) * B + k];
+#pragma unroll
+        for (int p = 0; p < 9; ++p) row[p] = sGF[p * 64 + l0 + c16];
+        staticFor<nl>([&](auto jI) {
+          constexpr int jj = decltype(jI)::value;
+          fmacRowBcast9<jj>(acc[r], row, xv[jj]);
+        });
+      });
     }
   }
   if (!any) return;
   // all old values loaded first (clamped rows, in bounds), then stored
   real* const q = dofs + e * lay.qStride;
-  for (int r = 0; r < (B + kDrThreads - 1) / kDrThreads; ++r) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
     const int k = tid + r * kDrThreads;
     real old[9];
 #pragma unroll
