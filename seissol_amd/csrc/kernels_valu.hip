@@ -135,105 +135,191 @@ __device__ inline void applyStars(const real (&acc)[3][R][9], const real* sS, re
 }
 
 // acc[j][r][p] += sum_{l<K} AT_j[l][row_r] * X[l][p], j < J, rows < M.
-// Mode 2: two steps per iteration with two register sets, the entries of
-// steps l + 2 and l + 3 loaded while steps l and l + 1 are computed, so that
-// a wave does not wait on the cache once per step; else one step at a time,
-// unrolled as the compiler chooses (mode 0) or not unrolled (mode 1).
-template <int J, int R, int Mode, typename real>
+// AheadX: both operands of a step are loaded ahead of it, the matrix entries
+// (L1/L2) and the step's row of X (LDS); else X is read at its step, and in
+// modes 0 and 1 the matrix entries too.  Mode 2: two steps per iteration with
+// two register sets, the operands of steps l + 2 and l + 3 loaded while steps
+// l and l + 1 are computed; else one step at a time, unrolled as the compiler
+// chooses (mode 0) or not unrolled (mode 1).
+template <int J, int R, int Mode, typename real, bool AheadX>
 __device__ inline void gemmRows(real (&acc)[J][R][9], const real* __restrict__ AT, int64_t strideJ, int ldAT,
                                 const real* X, int K, int t, int tpe, int M) {
-  if (K <= 0) return;
-  if constexpr (Mode == 1) {
+  if constexpr (!AheadX) {
+    if (K <= 0) return;
+    if constexpr (Mode == 1) {
 #pragma unroll 1
-    for (int l = 0; l < K; ++l) {
+      for (int l = 0; l < K; ++l) {
+        real x[9];
+#pragma unroll
+        for (int p = 0; p < 9; ++p) x[p] = X[l * 9 + p];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int row = t + r * tpe;
+            const int rr = row < M ? row : M - 1;
+            const real av = AT[j * strideJ + static_cast<int64_t>(l) * ldAT + rr];
+            const real a = row < M ? av : real(0);
+#pragma unroll
+            for (int p = 0; p < 9; ++p) acc[j][r][p] += a * x[p];
+          }
+        }
+      }
+      return;
+    }
+    if constexpr (Mode == 0) {
+      for (int l = 0; l < K; ++l) {
+        real x[9];
+#pragma unroll
+        for (int p = 0; p < 9; ++p) x[p] = X[l * 9 + p];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int row = t + r * tpe;
+            // clamped address: lanes past the last row read a valid entry and drop it
+            const int rr = row < M ? row : M - 1;
+            const real av = AT[j * strideJ + static_cast<int64_t>(l) * ldAT + rr];
+            const real a = row < M ? av : real(0);
+#pragma unroll
+            for (int p = 0; p < 9; ++p) acc[j][r][p] += a * x[p];
+          }
+        }
+      }
+      return;
+    }
+    int rr[R];
+    bool ok[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = t + r * tpe;
+      // clamped address: lanes past the last row read a valid entry and drop it
+      ok[r] = row < M;
+      rr[r] = ok[r] ? row : M - 1;
+    }
+    auto load = [&](int l, real (&v)[J][R]) {
+      const int lc = l < K ? l : K - 1;  // steps past the end reload the last one (in bounds, unused)
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[j][r] = AT[j * strideJ + static_cast<int64_t>(lc) * ldAT + rr[r]];
+    };
+    auto step = [&](int l, const real (&v)[J][R]) {
       real x[9];
 #pragma unroll
       for (int p = 0; p < 9; ++p) x[p] = X[l * 9 + p];
 #pragma unroll
-      for (int j = 0; j < J; ++j) {
+      for (int j = 0; j < J; ++j)
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int row = t + r * tpe;
-          const int rr = row < M ? row : M - 1;
-          const real av = AT[j * strideJ + static_cast<int64_t>(l) * ldAT + rr];
-          const real a = row < M ? av : real(0);
+          const real a = ok[r] ? v[j][r] : real(0);
 #pragma unroll
           for (int p = 0; p < 9; ++p) acc[j][r][p] += a * x[p];
         }
-      }
-    }
-    return;
-  }
-  if constexpr (Mode == 0) {
-    for (int l = 0; l < K; ++l) {
-      real x[9];
+    };
+    real a0[J][R], a1[J][R];
+    load(0, a0);
+    load(1, a1);
+    int l = 0;
+    for (; l + 1 < K; l += 2) {
+      real b0[J][R], b1[J][R];
+      load(l + 2, b0);
+      load(l + 3, b1);
+      __builtin_amdgcn_sched_barrier(0);
+      step(l, a0);
+      step(l + 1, a1);
 #pragma unroll
-      for (int p = 0; p < 9; ++p) x[p] = X[l * 9 + p];
-#pragma unroll
-      for (int j = 0; j < J; ++j) {
+      for (int j = 0; j < J; ++j)
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int row = t + r * tpe;
-          // clamped address: lanes past the last row read a valid entry and drop it
-          const int rr = row < M ? row : M - 1;
-          const real av = AT[j * strideJ + static_cast<int64_t>(l) * ldAT + rr];
-          const real a = row < M ? av : real(0);
+          a0[j][r] = b0[j][r];
+          a1[j][r] = b1[j][r];
+        }
+    }
+    if (l < K) step(l, a0);
+  } else {
+    if (K <= 0) return;
+    int rr[R];
+    bool ok[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = t + r * tpe;
+      // clamped address: lanes past the last row read a valid entry and drop it
+      ok[r] = row < M;
+      rr[r] = ok[r] ? row : M - 1;
+    }
+    // steps past the end reload the last one (in bounds, unused)
+    auto load = [&](int l, real (&v)[J][R]) {
+      const int lc = l < K ? l : K - 1;
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[j][r] = AT[j * strideJ + static_cast<int64_t>(lc) * ldAT + rr[r]];
+    };
+    auto loadX = [&](int l, real (&x)[9]) {
+      const int lc = l < K ? l : K - 1;
+#pragma unroll
+      for (int p = 0; p < 9; ++p) x[p] = X[lc * 9 + p];
+    };
+    auto step = [&](const real (&x)[9], const real (&v)[J][R]) {
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const real a = ok[r] ? v[j][r] : real(0);
 #pragma unroll
           for (int p = 0; p < 9; ++p) acc[j][r][p] += a * x[p];
         }
+    };
+    auto copy = [](auto& dst, const auto& src) {
+      constexpr int N = sizeof(src) / sizeof(real);
+      const real* s = reinterpret_cast<const real*>(&src);
+      real* d = reinterpret_cast<real*>(&dst);
+#pragma unroll
+      for (int i = 0; i < N; ++i) d[i] = s[i];
+    };
+    if constexpr (Mode == 2) {
+      real a0[J][R], a1[J][R], x0[9], x1[9];
+      load(0, a0);
+      load(1, a1);
+      loadX(0, x0);
+      loadX(1, x1);
+      int l = 0;
+      for (; l + 1 < K; l += 2) {
+        real b0[J][R], b1[J][R], y0[9], y1[9];
+        load(l + 2, b0);
+        load(l + 3, b1);
+        loadX(l + 2, y0);
+        loadX(l + 3, y1);
+        __builtin_amdgcn_sched_barrier(0);
+        step(x0, a0);
+        step(x1, a1);
+        copy(a0, b0);
+        copy(a1, b1);
+        copy(x0, y0);
+        copy(x1, y1);
+      }
+      if (l < K) step(x0, a0);
+    } else {
+      real a0[J][R], x0[9];
+      load(0, a0);
+      loadX(0, x0);
+      auto body = [&](int l) {
+        real b0[J][R], y0[9];
+        load(l + 1, b0);
+        loadX(l + 1, y0);
+        step(x0, a0);
+        copy(a0, b0);
+        copy(x0, y0);
+      };
+      if constexpr (Mode == 1) {
+#pragma unroll 1
+        for (int l = 0; l < K; ++l) body(l);
+      } else {
+        for (int l = 0; l < K; ++l) body(l);
       }
     }
-    return;
   }
-  int rr[R];
-  bool ok[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int row = t + r * tpe;
-    // clamped address: lanes past the last row read a valid entry and drop it
-    ok[r] = row < M;
-    rr[r] = ok[r] ? row : M - 1;
-  }
-  auto load = [&](int l, real (&v)[J][R]) {
-    const int lc = l < K ? l : K - 1;  // steps past the end reload the last one (in bounds, unused)
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[j][r] = AT[j * strideJ + static_cast<int64_t>(lc) * ldAT + rr[r]];
-  };
-  auto step = [&](int l, const real (&v)[J][R]) {
-    real x[9];
-#pragma unroll
-    for (int p = 0; p < 9; ++p) x[p] = X[l * 9 + p];
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const real a = ok[r] ? v[j][r] : real(0);
-#pragma unroll
-        for (int p = 0; p < 9; ++p) acc[j][r][p] += a * x[p];
-      }
-  };
-  real a0[J][R], a1[J][R];
-  load(0, a0);
-  load(1, a1);
-  int l = 0;
-  for (; l + 1 < K; l += 2) {
-    real b0[J][R], b1[J][R];
-    load(l + 2, b0);
-    load(l + 3, b1);
-    __builtin_amdgcn_sched_barrier(0);
-    step(l, a0);
-    step(l + 1, a1);
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        a0[j][r] = b0[j][r];
-        a1[j][r] = b1[j][r];
-      }
-  }
-  if (l < K) step(l, a0);
 }
 
 // product loop of gemmRows per kernel, each choice measured faster than the
@@ -296,6 +382,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
   using S = Shape<O, RX, WV>;
   constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
   constexpr int kProd = productMode<real, O, RX, false>();
+  constexpr bool kAheadX = false;  // X loaded at the step (FP64: ahead it spills; FP32: no faster)
   __shared__ real sA[S::EPB * S::QS];
   __shared__ real sB[S::EPB * (S::FS > S::QS ? S::FS : S::QS)];
   __shared__ real sStar[S::EPB * 3 * kStarNnz];
@@ -359,7 +446,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int p = 0; p < 9; ++p) acc[j][r][p] = real(0);
-    gemmRows<3, R, kProd>(acc, mats + S::ck, static_cast<int64_t>(B) * B, B, src, nIn, t, TPE, m);
+    gemmRows<3, R, kProd, real, kAheadX>(acc, mats + S::ck, static_cast<int64_t>(B) * B, B, src, nIn, t, TPE, m);
     real dq[R][9];
     applyStars<R>(acc, sS, dq);
     __syncthreads();  // all reads of dst (previous src) done
@@ -419,7 +506,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
         for (int r = 0; r < R; ++r)
 #pragma unroll
           for (int p = 0; p < 9; ++p) acc[j][r][p] = real(0);
-      gemmRows<3, R, kProd>(acc, mats + S::vol, static_cast<int64_t>(B) * B, B, xa, B, t, TPE, B);
+      gemmRows<3, R, kProd, real, kAheadX>(acc, mats + S::vol, static_cast<int64_t>(B) * B, B, xa, B, t, TPE, B);
       real y[R][9];
       applyStars<R>(acc, sS, y);
 #pragma unroll
@@ -435,7 +522,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int p = 0; p < 9; ++p) tr[0][r][p] = real(0);
-      gemmRows<1, R, kProd>(tr, mats + S::fmrt + static_cast<int64_t>(f) * B * B2, 0, B2, xa, B, t, TPE, B2);
+      gemmRows<1, R, kProd, real, kAheadX>(tr, mats + S::fmrt + static_cast<int64_t>(f) * B * B2, 0, B2, xa, B, t, TPE, B2);
       const bool skip = !active || (faceTypes != nullptr && faceTypes[e * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE);
       const real* A = aplus + (active ? e : 0) * lay.aplusStride + 81 * f;
 #pragma unroll
@@ -460,7 +547,7 @@ __global__ __launch_bounds__(64 * WV) void kAder(int64_t n, const real* __restri
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int p = 0; p < 9; ++p) acc[0][r][p] = qreg[r][p];
-      gemmRows<1, R, kProd>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
+      gemmRows<1, R, kProd, real, kAheadX>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
       if (active) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -485,6 +572,7 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
   using S = Shape<O>;
   constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
   constexpr int kProd = productMode<real, O, 0, false>();
+  constexpr bool kAheadX = false;  // X loaded at the step (FP64: ahead it spills; FP32: no faster)
   __shared__ real sA[S::EPB * S::QS];
   __shared__ real sB[S::EPB * S::FS];
   __shared__ real sStar[S::EPB * 3 * kStarNnz];
@@ -519,7 +607,7 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int p = 0; p < 9; ++p) acc[j][r][p] = real(0);
-    gemmRows<3, R, kProd>(acc, mats + S::vol, static_cast<int64_t>(B) * B, B, xa, B, t, TPE, B);
+    gemmRows<3, R, kProd, real, kAheadX>(acc, mats + S::vol, static_cast<int64_t>(B) * B, B, xa, B, t, TPE, B);
     real y[R][9];
     applyStars<R>(acc, sS, y);
 #pragma unroll
@@ -534,7 +622,7 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
     for (int r = 0; r < R; ++r)
 #pragma unroll
       for (int p = 0; p < 9; ++p) tr[0][r][p] = real(0);
-    gemmRows<1, R, kProd>(tr, mats + S::fmrt + static_cast<int64_t>(f) * B * B2, 0, B2, xa, B, t, TPE, B2);
+    gemmRows<1, R, kProd, real, kAheadX>(tr, mats + S::fmrt + static_cast<int64_t>(f) * B * B2, 0, B2, xa, B, t, TPE, B2);
     const bool skip = !active || (faceTypes != nullptr && faceTypes[e * 4 + f] == SDG_FACE_DYNAMIC_RUPTURE);
     const real* A = aplus + (active ? e : 0) * lay.aplusStride + 81 * f;
 #pragma unroll
@@ -558,7 +646,7 @@ __global__ __launch_bounds__(256) void kLocal(int64_t n, const real* __restrict_
   for (int r = 0; r < R; ++r)
 #pragma unroll
     for (int p = 0; p < 9; ++p) acc[0][r][p] = qreg[r][p];
-  gemmRows<1, R, kProd>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
+  gemmRows<1, R, kProd, real, kAheadX>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
   if (active) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -584,6 +672,9 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
   using S = Shape<O, RX, WV>;
   constexpr int B = S::B, B2 = S::B2, R = S::R, TPE = S::TPE;
   constexpr int kProd = productMode<real, O, RX, true>();
+  // FP64: X loaded ahead (the neighbour kernels need 130 fewer registers at
+  // ORDER 5); FP32: at the step (ahead measured slower at ORDER 6)
+  constexpr bool kAheadX = sizeof(real) == 8;
   __shared__ real sA[S::EPB * S::QS];
   __shared__ real sU[S::EPB * B2 * 9];
   __shared__ real sB[S::EPB * S::FS];
@@ -684,7 +775,7 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int p = 0; p < 9; ++p) u[0][r][p] = real(0);
-      gemmRows<1, R, kProd>(u, mats + S::rt + static_cast<int64_t>(j) * B * B2, 0, B2, xa, B, t, TPE, B2);
+      gemmRows<1, R, kProd, real, kAheadX>(u, mats + S::rt + static_cast<int64_t>(j) * B * B2, 0, B2, xa, B, t, TPE, B2);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int row = t + r * TPE;
@@ -700,7 +791,7 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int p = 0; p < 9; ++p) v[0][r][p] = real(0);
-      gemmRows<1, R, kProd>(v, mats + S::fp + static_cast<int64_t>(h) * B2 * B2, 0, B2, xu, B2, t, TPE, B2);
+      gemmRows<1, R, kProd, real, kAheadX>(v, mats + S::fp + static_cast<int64_t>(h) * B2 * B2, 0, B2, xu, B2, t, TPE, B2);
       const real* A = aminus + (active ? e : 0) * lay.aminusStride + 81 * f;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -724,7 +815,7 @@ __global__ __launch_bounds__(64 * WV) void kNeighbor(int64_t n, const real* __re
   for (int r = 0; r < R; ++r)
 #pragma unroll
     for (int p = 0; p < 9; ++p) acc[0][r][p] = real(0);
-  gemmRows<1, R, kProd>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
+  gemmRows<1, R, kProd, real, kAheadX>(acc, mats + S::rdm, 0, B, xb, 4 * B2, t, TPE, B);
   if (active) {
     // all old values loaded first (clamped rows, in bounds), then stored
     real* const q = dofs + e * lay.qStride;
