@@ -30,7 +30,8 @@ def _case(order, seed, uniform=False, points=None):
 
 
 @pytest.mark.parametrize("order,arith,tol,extra", [
-    (3, "fp64", 1e-12, 0), (5, "fp64", 1e-12, 3), (6, "fp64", 1e-12, 0), (4, "fp32", 2e-5, 0), (6, "fp32", 2e-5, 0),
+    (2, "fp64", 1e-12, 0), (3, "fp64", 1e-12, 0), (5, "fp64", 1e-12, 3), (6, "fp64", 1e-12, 0), (7, "fp64", 1e-12, 0),
+    (2, "fp32", 2e-5, 2), (4, "fp32", 2e-5, 0), (6, "fp32", 2e-5, 0), (7, "fp32", 2e-5, 0),
 ])
 def test_space_time_interpolation_matches_oracle(gpu, order, arith, tol, extra):
     import torch
@@ -59,7 +60,8 @@ def test_space_time_interpolation_matches_oracle(gpu, order, arith, tol, extra):
     assert ep < tol and em < tol
 
 
-@pytest.mark.parametrize("order,arith,tol", [(3, "fp64", 1e-12), (6, "fp64", 1e-12), (5, "fp32", 2e-5)])
+@pytest.mark.parametrize("order,arith,tol", [(2, "fp64", 1e-12), (3, "fp64", 1e-12), (6, "fp64", 1e-12),
+                                             (7, "fp64", 1e-12), (5, "fp32", 2e-5), (7, "fp32", 2e-5)])
 def test_nodal_flux_matches_oracle(gpu, order, arith, tol):
     import torch
     setup, faces = _case(order, seed=10 + order)
