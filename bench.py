@@ -268,8 +268,9 @@ def lts_bench(args) -> dict:
         "value": out["graph"]["element_updates_per_s"],
         "graph_vs_eager": out["eager_autotuned"]["ms_per_macro_step"] / out["graph"]["ms_per_macro_step"], "unit": "element-updates/s", "higher_is_better": True,
         "modes": out, "tiles": {str(c): t for c, t in run.tiles().items()},
-        "tile_variants": ("VALU: bit 0 one more basis row per lane, bit 1 two waves per block; matrix-core "
-                          "engines, local kernel: 0 = matrix-core pass, 1 + v = FP32 VALU variant v"),
+        "tile_variants": ("VALU 0-3: bit 0 one more basis row per lane, bit 1 two waves per block; neighbour "
+                          "4: one element per 16-lane row; matrix-core engines, local kernel: 0 = matrix-core "
+                          "pass, 1 + v = FP32 VALU variant v"),
         "dtype": {"fp64": "f64", "fp32": "f32"}.get(args.lts_form,
                                                    f"f32 ({args.lts_form} matrix-core local pass, FP32 neighbour)"),
         "config": {"workload": f"ORDER {args.lts_order} elastic, {lts.setup.n} tetrahedra (periodic box "

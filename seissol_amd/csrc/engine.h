@@ -100,6 +100,10 @@ int launchTaylor(const EngineState& e, const double* coeffs, const void* derivat
                  hipStream_t stream);
 // local time stepping kernels; `variant` selects a tile shape (0..3, 0 = default)
 constexpr int kTileVariants = 4;
+// neighbour pass: the kTileVariants tiles of kNeighbor, then kNeighborRows
+// (one element per 16-lane row)
+constexpr int kNeighborRowsVariant = kTileVariants;
+constexpr int kNeighborVariants = kTileVariants + 1;
 int launchLocalIntegration(const EngineState& e, const double* coeffs, const sdg_layer* layer, int resetBuffers,
                            hipStream_t stream, int variant = 0);
 int launchNeighboringIntegration(const EngineState& e, const double* timeCoeffs, const double* subtimeCoeffs,

@@ -56,10 +56,16 @@ bool ltsLayerOk(const sdg_layer* L, bool neighbors, const char* who) {
 // Local-pass variants: the FP64/FP32 engines have the kTileVariants VALU tile
 // shapes; a matrix-core engine has its matrix-core pass (variant 0) and the
 // FP32 VALU shapes after it.  The neighbour pass is the VALU kernel of the
-// engine's real type (FP32 for the matrix-core engines) with kTileVariants shapes.
+// engine's real type (FP32 for the matrix-core engines) with kTileVariants
+// shapes, then the one-element-per-row kernel (kNeighborRowsVariant).
 // The FP64 engine (orders 3-7, kernel family AUTO) likewise has its FP64
 // matrix-core pass as variant 0.
 bool matrixCoreLocal(const EngineState& e) { return matrixCore(e.arithmetic) || hasMfma64(e); }
+// Neighbour-pass variant before any tuning: one element per 16-lane row for
+// the FP32 kernel at orders 3-6 (measured faster there), else tile 0.
+int defaultNeighborVariant(const EngineState& e) {
+  return (e.arithmetic != SDG_FP64 && e.order >= 3 && e.order <= 6) ? kNeighborRowsVariant : 0;
+}
 int localVariants(const EngineState& e) { return matrixCoreLocal(e) ? 1 + kTileVariants : kTileVariants; }
 
 int runLocal(const EngineState& e, const double* coeffs, const sdg_layer* own, int reset, hipStream_t s, int v) {
@@ -148,7 +154,7 @@ extern "C" int sdg_compute_neighboring_integration(sdg_engine* engine, const dou
       !layerStridesOk(engine->s, layer, "sdg_compute_neighboring_integration"))
     return SDG_ERR_INVALID;
   return launchNeighboringIntegration(engine->s, time_coeffs, subtime_coeffs, layer,
-                                      static_cast<hipStream_t>(stream));
+                                      static_cast<hipStream_t>(stream), defaultNeighborVariant(engine->s));
 }
 
 struct sdg_lts {
@@ -254,7 +260,7 @@ extern "C" int sdg_lts_create(sdg_lts** out, sdg_engine* engine, const sdg_layer
   L->start.assign(cluster_start, cluster_start + num_clusters + 1);
   L->dt0 = dt0;
   L->tileLocal.assign(num_clusters, 0);
-  L->tileNeighbor.assign(num_clusters, 0);
+  L->tileNeighbor.assign(num_clusters, defaultNeighborVariant(engine->s));
   *out = L;
   return SDG_OK;
 }
@@ -303,7 +309,7 @@ extern "C" int sdg_lts_autotune(sdg_lts* lts, int repeats, void* stream) {
     for (int kind = 0; kind < 2 && rc == SDG_OK; ++kind) {
       float best = 0.0f;
       int bestV = 0;
-      const int variants = kind == 0 ? localVariants(lts->engine->s) : kTileVariants;
+      const int variants = kind == 0 ? localVariants(lts->engine->s) : kNeighborVariants;
       for (int v = 0; v < variants && rc == SDG_OK; ++v) {
         (void)hipEventRecord(e0, st);
         for (int r = 0; r < repeats && rc == SDG_OK; ++r)
@@ -351,7 +357,7 @@ extern "C" int sdg_lts_autotune(sdg_lts* lts, int repeats, void* stream) {
 
 extern "C" int sdg_lts_set_tile_variant(sdg_lts* lts, int cluster, int kernel, int variant) {
   if (lts == nullptr || cluster < 0 || cluster >= static_cast<int>(lts->tileLocal.size()) || kernel < 0 ||
-      kernel > 1 || variant < 0 || variant >= (kernel == 0 ? localVariants(lts->engine->s) : kTileVariants)) {
+      kernel > 1 || variant < 0 || variant >= (kernel == 0 ? localVariants(lts->engine->s) : kNeighborVariants)) {
     setError("sdg_lts_set_tile_variant: invalid arguments");
     return SDG_ERR_INVALID;
   }

@@ -51,6 +51,28 @@ def test_lts_matches_oracle(gpu, order, fractions, arith, tol):
     assert err < tol
 
 
+@pytest.mark.parametrize("order,arith,tol", [
+    (2, "fp64", 1e-11), (3, "fp64", 1e-11), (4, "fp32", 2e-5), (6, "fp64", 1e-11), (6, "fp32", 2e-5),
+    (7, "fp64", 1e-11), (7, "fp32", 2e-5), (8, "fp32", 2e-5),
+])
+def test_neighbour_rows_variant_matches_oracle(gpu, order, arith, tol):
+    # the neighbour pass with one element per 16-lane row (tile variant 4)
+    # at the orders the LTS tile tests above leave out
+    import torch
+    lts = make_lts_setup(order, fractions=(0.5, 0.3, 0.2), dims=(8, 6, 6), seed=11)
+    print(f"neighbour rows order={order} {arith} n={lts.setup.n}", flush=True)
+    eng = Engine(order, arith)
+    layer = DeviceLtsLayer(eng, lts)
+    run = LocalTimeStepping(eng, layer)
+    run.set_tile(4, kernel=1)
+    run.advance(2)
+    torch.cuda.synchronize()
+    q_ref = _oracle_q(lts, order, 2, layer)
+    err = rel_l2(_np(layer.dofs).astype(np.float64), q_ref)
+    print(f"rel L2 vs oracle after 2 macro steps: {err:.2e}")
+    assert err < tol
+
+
 @pytest.mark.parametrize("arith", ["fp64", "fp32"])
 def test_every_tile_variant_and_the_autotuner_keep_the_results(gpu, arith):
     import torch
@@ -73,6 +95,17 @@ def test_every_tile_variant_and_the_autotuner_keep_the_results(gpu, arith):
         err = rel_l2(_np(layer.dofs).astype(np.float64), q_ref)
         print(f"{arith} tile variant {variant}: rel L2 vs oracle {err:.2e}", flush=True)
         assert err < tol
+    # the neighbour pass with one element per 16-lane row
+    layer = DeviceLtsLayer(eng, lts)
+    run = LocalTimeStepping(eng, layer)
+    run.set_tile(4, kernel=1)
+    run.advance(2)
+    torch.cuda.synchronize()
+    err = rel_l2(_np(layer.dofs).astype(np.float64), q_ref)
+    print(f"{arith} neighbour variant 4: rel L2 vs oracle {err:.2e}", flush=True)
+    assert err < tol
+    with pytest.raises(_capi.SdgError):
+        run.set_tile(5, kernel=1)
     # the autotuner restores the state it ran its trials on
     layer = DeviceLtsLayer(eng, lts)
     run = LocalTimeStepping(eng, layer)
@@ -81,7 +114,7 @@ def test_every_tile_variant_and_the_autotuner_keep_the_results(gpu, arith):
     torch.cuda.synchronize()
     assert bool(torch.equal(before, layer.dofs))
     assert set(tiles) == {0, 1, 2}
-    assert all(0 <= t["local"] < local_variants and 0 <= t["neigh"] < 4 for t in tiles.values())
+    assert all(0 <= t["local"] < local_variants and 0 <= t["neigh"] < 5 for t in tiles.values())
     run.advance(2, graph=True)
     torch.cuda.synchronize()
     assert rel_l2(_np(layer.dofs).astype(np.float64), q_ref) < tol
@@ -149,13 +182,22 @@ def test_matrix_core_lts_variants_and_autotuner(gpu):
         err = rel_l2(_np(layer.packed(layer.dofs)).astype(np.float64), q_ref)
         print(f"f16x3 local variant {variant}: rel L2 vs oracle {err:.2e}", flush=True)
         assert err < 1e-5
+    # the FP32 neighbour pass with one element per 16-lane row
+    layer = DeviceLtsLayer(eng, lts)
+    run = LocalTimeStepping(eng, layer)
+    run.set_tile(4, kernel=1)
+    run.advance(2)
+    torch.cuda.synchronize()
+    err = rel_l2(_np(layer.packed(layer.dofs)).astype(np.float64), q_ref)
+    print(f"f16x3 neighbour variant 4: rel L2 vs oracle {err:.2e}", flush=True)
+    assert err < 1e-5
     layer = DeviceLtsLayer(eng, lts)
     run = LocalTimeStepping(eng, layer)
     before = layer.dofs.clone()
     tiles = run.autotune(repeats=2)
     torch.cuda.synchronize()
     assert bool(torch.equal(before, layer.dofs))
-    assert all(0 <= t["local"] < 5 and 0 <= t["neigh"] < 4 for t in tiles.values())
+    assert all(0 <= t["local"] < 5 and 0 <= t["neigh"] < 5 for t in tiles.values())
     with pytest.raises(_capi.SdgError):
         run.set_tile(5, kernel=0)
     run.advance(2, graph=True)
