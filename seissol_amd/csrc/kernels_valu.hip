@@ -910,6 +910,17 @@ __global__ __launch_bounds__(64) void kNeighborRows(int64_t n, const real* __res
     staticFor<(K + 15) / 16>([&](auto runI) {
       constexpr int l0 = 16 * decltype(runI)::value;
       constexpr int nl = K - l0 < 16 ? K - l0 : 16;
+      // FP32: the run's matrix entries all loaded before its first FMA (the
+      // compiler does not move loads across the FMA statements: loaded at
+      // their step, each step waits on its load); FP64: at the step (ahead
+      // they cost the second wave per SIMD)
+      constexpr bool kRunAhead = sizeof(real) == 4;
+      real av[kRunAhead ? nl : 1][R];
+      if constexpr (kRunAhead)
+#pragma unroll
+        for (int jj = 0; jj < nl; ++jj)
+#pragma unroll
+          for (int r = 0; r < R; ++r) av[jj][r] = AT[static_cast<int64_t>(l0 + jj) * ldAT + rr[r]];
       real row[9];
 #pragma unroll
       for (int p = 0; p < 9; ++p) row[p] = Xs[(l0 + c) * 9 + p];
@@ -917,8 +928,12 @@ __global__ __launch_bounds__(64) void kNeighborRows(int64_t n, const real* __res
         constexpr int jj = decltype(jI)::value;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const real av = AT[static_cast<int64_t>(l0 + jj) * ldAT + rr[r]];
-          fmacRowBcast9<jj>(acc[r], row, ok[r] ? av : real(0));
+          real a;
+          if constexpr (kRunAhead)
+            a = av[jj][r];
+          else
+            a = AT[static_cast<int64_t>(l0 + jj) * ldAT + rr[r]];
+          fmacRowBcast9<jj>(acc[r], row, ok[r] ? a : real(0));
         }
       });
     });
