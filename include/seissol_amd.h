@@ -241,7 +241,8 @@ int sdg_lts_autotune(sdg_lts* lts, int repeats, void* stream);
 /* Tile variant in use for `cluster` and kernel 0 (local) / 1 (neighbour).
  * VALU variants 0-3: bit 0 = one more basis row per lane, bit 1 = two waves
  * per block; neighbour variant 4 = one element per 16-lane row (the default
- * of the FP32 neighbour pass at orders 3-6, else 0).  Matrix-core engines,
+ * of the FP32 neighbour pass at orders 3-6 and of the FP64 one at orders
+ * 4-5, else 0).  Matrix-core engines,
  * local kernel: 0 = the matrix-core pass, 1 + v = the FP32 VALU variant v. */
 int sdg_lts_tile_variant(const sdg_lts* lts, int cluster, int kernel);
 int sdg_lts_set_tile_variant(sdg_lts* lts, int cluster, int kernel, int variant);

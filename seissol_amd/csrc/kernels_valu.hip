@@ -882,12 +882,14 @@ __global__ __launch_bounds__(64) void kNeighborRows(int64_t n, const real* __res
   }
   const unsigned setupAll = (lts.setup != nullptr && active) ? lts.setup[e] : 0u;
   auto pick = [](const auto (&v)[4], int f) { return f == 0 ? v[0] : (f == 1 ? v[1] : (f == 2 ? v[2] : v[3])); };
-  // FP32 at ORDER 5: matrix entries loaded in groups of 8 steps and the
-  // neighbour buffer at its face, which takes the kernel to 160 registers
-  // and three waves per SIMD (measured 9 % faster there; slower at orders 4
-  // and 6, which keep groups of 16 and the buffer one face ahead)
+  // the matrix entries of a product are loaded in groups of kGroup steps.
+  // FP32 at ORDER 5: groups of 8 and the neighbour buffer at its face, which
+  // takes the kernel to 160 registers and three waves per SIMD (measured 9 %
+  // faster there; slower at orders 4 and 6, which keep groups of 16 and the
+  // buffer one face ahead).  FP64: groups of 4 (16 cost the second wave per
+  // SIMD; 4 measured 11 % faster than loading each entry at its step)
   constexpr bool kLean = sizeof(real) == 4 && O == 5;
-  constexpr int kGroup = kLean ? 8 : 16;
+  constexpr int kGroup = sizeof(real) == 8 ? 4 : (kLean ? 8 : 16);
   // the neighbour buffer of a face gathered into registers (clamped
   // addresses; FP32: one face ahead, in flight during the previous face)
   constexpr bool kAhead = sizeof(real) == 4 && !kLean;
@@ -916,18 +918,16 @@ __global__ __launch_bounds__(64) void kNeighborRows(int64_t n, const real* __res
     staticFor<(K + 15) / 16>([&](auto runI) {
       constexpr int l0 = 16 * decltype(runI)::value;
       constexpr int nl = K - l0 < 16 ? K - l0 : 16;
-      // FP32: the matrix entries of a group of H steps all loaded before its
+      // the matrix entries of a group of kGroup steps all loaded before its
       // first FMA (the compiler does not move loads across the FMA
-      // statements: loaded at their step, each step waits on its load);
-      // FP64: at the step (ahead they cost the second wave per SIMD)
-      constexpr bool kRunAhead = sizeof(real) == 4;
-      real av[kRunAhead ? kGroup : 1][R];
+      // statements: loaded at their step, each step waits on its load)
+      real av[kGroup][R];
       real row[9];
 #pragma unroll
       for (int p = 0; p < 9; ++p) row[p] = Xs[(l0 + c) * 9 + p];
       staticFor<nl>([&](auto jI) {
         constexpr int jj = decltype(jI)::value;
-        if constexpr (kRunAhead && jj % kGroup == 0) {
+        if constexpr (jj % kGroup == 0) {
 #pragma unroll
           for (int u = 0; u < kGroup; ++u)
             if (jj + u < nl)
@@ -935,14 +935,7 @@ __global__ __launch_bounds__(64) void kNeighborRows(int64_t n, const real* __res
               for (int r = 0; r < R; ++r) av[u][r] = AT[static_cast<int64_t>(l0 + jj + u) * ldAT + rr[r]];
         }
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          real a;
-          if constexpr (kRunAhead)
-            a = av[jj % kGroup][r];
-          else
-            a = AT[static_cast<int64_t>(l0 + jj) * ldAT + rr[r]];
-          fmacRowBcast9<jj>(acc[r], row, ok[r] ? a : real(0));
-        }
+        for (int r = 0; r < R; ++r) fmacRowBcast9<jj>(acc[r], row, ok[r] ? av[jj % kGroup][r] : real(0));
       });
     });
   };

@@ -62,9 +62,12 @@ bool ltsLayerOk(const sdg_layer* L, bool neighbors, const char* who) {
 // matrix-core pass as variant 0.
 bool matrixCoreLocal(const EngineState& e) { return matrixCore(e.arithmetic) || hasMfma64(e); }
 // Neighbour-pass variant before any tuning: one element per 16-lane row for
-// the FP32 kernel at orders 3-6 (measured faster there), else tile 0.
+// the FP32 kernel at orders 3-6 and the FP64 kernel at orders 4-5 (where
+// the autotuner picked it on every cluster), else tile 0.
 int defaultNeighborVariant(const EngineState& e) {
-  return (e.arithmetic != SDG_FP64 && e.order >= 3 && e.order <= 6) ? kNeighborRowsVariant : 0;
+  const bool fp64 = e.arithmetic == SDG_FP64;
+  const bool rows = fp64 ? (e.order >= 4 && e.order <= 5) : (e.order >= 3 && e.order <= 6);
+  return rows ? kNeighborRowsVariant : 0;
 }
 int localVariants(const EngineState& e) { return matrixCoreLocal(e) ? 1 + kTileVariants : kTileVariants; }
 
