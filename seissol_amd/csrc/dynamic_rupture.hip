@@ -80,7 +80,6 @@ constexpr int kDrWaves = 4;   // sides in flight per interpolation workgroup (on
 constexpr int kDrScan = 1;    // faces of the chunk per thread
 constexpr int kDrChunk = 64 * kDrWaves * kDrScan;  // faces scanned by one interpolation workgroup
 
-
 // position of entry i of the packed derivatives (blocks [9][B(O-d)] one
 // after another) in the layout's derivatives (blocks [9][rows(B(O-d))])
 template <int O>
@@ -118,7 +117,9 @@ __host__ __device__ constexpr int columnOffset(int d) {
 // k = Stroud point), and combined at the time points,
 // out[f][o][p][k] = sum_d c[o][d] w_d[k][p] (FP64, the friction law's
 // q_interpolated layout; points k in [(O+1)^2, P) are written as zero).
-// The gather of a wave's next side is issued before the arithmetic of its
+// The side's factors shared by all points (the derivatives, the rotation)
+// are read once per 16-lane row and broadcast with DPP row_newbcast.  The
+// gather of a wave's next side is issued before the arithmetic of its
 // current one, so that the loads are in flight while it computes.
 template <typename real, int O>
 __global__ __launch_bounds__(64 * kDrWaves, 2) void kDrInterpolate(int64_t nf, const real* __restrict__ V,
