@@ -299,6 +299,27 @@ def test_f16x3_stays_within_1e5_of_the_oracle_over_200_steps(gpu):
     assert max(per_q) < 1e-5, per_q
 
 
+@pytest.mark.parametrize("order", [3, 4, 5, 6])
+def test_fp32_neighbour_rows_with_dr_faces_match_oracle(gpu, order):
+    # FP32 at orders 3-6 runs the neighbour pass with one element per 16-lane
+    # row; DR faces in the layer take no neighbour flux
+    steps = 2
+    setup = make_setup(order, n=701, mesh="proxy", seed=43)
+    setup.face_types[::5, 1] = _capi.SDG_FACE_DYNAMIC_RUPTURE
+    print(f"fp32 neighbour rows order={order} n={setup.n}", flush=True)
+    eng = Engine(order, "fp32")
+    layer = DeviceLayer(eng, setup)
+    gm = global_matrices(order)["flat"]
+    coeffs = monomial_integrate(order, 0.0, setup.dt)
+    run_proxy(eng, layer, "all", steps, setup.dt)
+    _sync()
+    q_ref, _ = oracle.proxy_all(order, gm, steps, coeffs, _np(layer.star64), _np(layer.aplus64), _np(layer.aminus64),
+                                setup.neighbors, setup.face_relations, setup.dofs, setup.face_types)
+    err = rel_l2(_np(layer.dofs).astype(np.float64), q_ref)
+    print(f"fp32 order {order}: rel L2 after {steps} steps {err:.2e}")
+    assert err < 1e-5, err
+
+
 @pytest.mark.parametrize("kernels", ["auto", "valu"])
 @pytest.mark.parametrize("order", [3, 4, 5, 6, 7])
 def test_fp64_matrix_core_and_valu_passes_match_oracle(gpu, order, kernels):
