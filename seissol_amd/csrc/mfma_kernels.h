@@ -37,6 +37,7 @@
 #include <string>
 #include <type_traits>
 
+#include "dpp_rows.h"
 #include "engine.h"
 #include "mfma_common.h"
 
@@ -953,20 +954,25 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_N)) void kNei
           }
         }
       }
+      // W = U AminusT: rowA[p] holds A(min(rp, 8), p), broadcast across the
+      // element's 16 lanes (DPP row_newbcast) instead of read per (q, p)
       const float* A = sAm + sl * 81;
-      float w0[9], w1[9];
+      float rowA[9], w0[9], w1[9];
 #pragma unroll
       for (int p = 0; p < 9; ++p) {
-        float acc0 = 0.0f, acc1 = 0.0f;
-        if (regular)
+        rowA[p] = A[min(rp, 8) + 9 * p];
+        w0[p] = 0.0f;
+        w1[p] = 0.0f;
+      }
+      staticFor<9>([&](auto qI) {
+        constexpr int qq = decltype(qI)::value;
+        fmacRowBcast9<qq>(w0, rowA, u[0][qq]);
+        fmacRowBcast9<qq>(w1, rowA, u[1][qq]);
+      });
 #pragma unroll
-          for (int qq = 0; qq < 9; ++qq) {
-            const float a = A[qq + 9 * p];
-            acc0 += u[0][qq] * a;
-            acc1 += u[1][qq] * a;
-          }
-        w0[p] = acc0;
-        w1[p] = acc1;
+      for (int p = 0; p < 9; ++p) {
+        w0[p] = regular ? w0[p] : 0.0f;
+        w1[p] = regular ? w1[p] : 0.0f;
       }
       if constexpr (scaled) {
         // column maxima over the 16 lanes of the element (idle lanes hold zeros)
