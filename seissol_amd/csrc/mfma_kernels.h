@@ -364,12 +364,13 @@ __device__ inline void updateRows(float* q, int ld, int row0, const floatx4 (&ac
 }
 
 // Y += sum_j (A_j X) star_j for the lane's element, with the matrix and
-// column scales divided out.  The products run in three passes over input
+// column scales divided out (the star values in LDS carry the CK matrices'
+// scales; kFactor: times aUn[j] as well).  The products run in three passes over input
 // quantity groups {6,7,8}, {0,1,2}, {3,4,5}, each holding 3 x 3 accumulator
 // tiles (the elastic star matrix maps velocities (6..8) to stresses (0..5)
 // and stresses to velocities); the three matrices' fragments of a k-step are
 // loaded inside the pass (L2-resident), which keeps them out of the live set.
-template <int V, int KSMAX, int LDX>
+template <int V, int KSMAX, int LDX, bool kFactor>
 __device__ inline void gemmStars(const uint16_t* const (&a)[3], int ks, const typename Prec<V>::T* xs, int lane,
                                  const float* sS, const float (&aUn)[3], const float (&colUn)[9], float (&y)[9][4],
                                  int passMask = 7) {
@@ -413,7 +414,8 @@ __device__ inline void gemmStars(const uint16_t* const (&a)[3], int ks, const ty
           if (k < kStarCnt[p]) {
             const int qi = kStarQ[p][k];
             if (qi >= q0 && qi < q0 + 3) {
-              const float sv = sS[jj * 24 + kStarOff[p] + k] * aUn[jj] * colUn[qi];
+              const float sv = kFactor ? sS[jj * 24 + kStarOff[p] + k] * aUn[jj] * colUn[qi]
+                                       : sS[jj * 24 + kStarOff[p] + k] * colUn[qi];
 #pragma unroll
               for (int j = 0; j < 4; ++j) y[p][j] += acc[jj][qi - q0][j] * sv;
             }
@@ -455,7 +457,9 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
   float* const tileTraces = traces + E0 * 36 * S::B2P;
   const float* aUn = tables + S::tAUn;
   const float aUnCk[3] = {aUn[0], aUn[1], aUn[2]};
-  const float aUnVol[3] = {aUn[3], aUn[4], aUn[5]};
+  // the volume products' factor relative to the staged star values (powers
+  // of two; 1 when kDivM's fragments carry the scales of kDivMT's)
+  const float aUnVol[3] = {aUn[3] / aUn[0], aUn[4] / aUn[1], aUn[5] / aUn[2]};
   const float aUnF = aUn[6], aUnR = aUn[7];
 
   {
@@ -481,10 +485,14 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
         sv[u] = star[el * starStride + 81 * (r / 24) + kStarPos[r % 24]];
       }
     }
+    // staged times the CK matrices' unscale factor of their direction (a
+    // power of two: exact), which the CK products then need not apply
 #pragma unroll
     for (int u = 0; u < SI; ++u) {
       const int i = threadIdx.x + u * S::THREADS;
-      if (i < kTileElems * 72) sStar[i] = E0 + i / 72 < n ? sv[u] : 0.0f;
+      const int jd = (i % 72) / 24;
+      const float f = jd == 0 ? aUnCk[0] : (jd == 1 ? aUnCk[1] : aUnCk[2]);
+      if (i < kTileElems * 72) sStar[i] = E0 + i / 72 < n ? sv[u] * f : 0.0f;
     }
   }
 
@@ -569,7 +577,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
       const uint16_t* a[3] = {frag + S::ck + (0LL * MT + tile) * S::KSB * kFragPair,
                               frag + S::ck + (1LL * MT + tile) * S::KSB * kFragPair,
                               frag + S::ck + (2LL * MT + tile) * S::KSB * kFragPair};
-      gemmStars<V, S::KSB, LDX>(a, ks, xs, lane, sS, aUnCk, un, dq[0], mask);
+      gemmStars<V, S::KSB, LDX, false>(a, ks, xs, lane, sS, aUnCk, un, dq[0], mask);
     }
     if (mode != 0) {
       if (slot >= 0)
@@ -707,7 +715,7 @@ __global__ __launch_bounds__(MShape<O>::THREADS, (Cfg<O, V>::WAVES_L)) void kLoc
     // `E0 < n` holds for every launched tile; the guard gives this GEMM (and
     // the trace GEMM above) a block of its own, with which the register
     // allocator keeps the kernel free of spills
-    if (E0 < n) gemmStars<V, S::KSB, LDX>(a, S::KSB, xs, lane, sS, aUnVol, un, vol);
+    if (E0 < n) gemmStars<V, S::KSB, LDX, true>(a, S::KSB, xs, lane, sS, aUnVol, un, vol);
   }
 
   // Once every wave has finished the volume GEMM, the X area is free until W
