@@ -205,6 +205,19 @@ def test_matrix_core_lts_variants_and_autotuner(gpu):
     assert rel_l2(_np(layer.packed(layer.dofs)).astype(np.float64), q_ref) < 1e-5
 
 
+@pytest.mark.parametrize("order,arith,neigh", [
+    (2, "fp32", 0), (3, "fp32", 4), (6, "fp32", 4), (7, "fp32", 0), (5, "f16x3", 4),
+    (3, "fp64", 0), (4, "fp64", 4), (5, "fp64", 4), (6, "fp64", 0),
+])
+def test_default_neighbour_tiles(gpu, order, arith, neigh):
+    # the untuned neighbour tile: one element per 16-lane row (4) where it was
+    # measured faster (FP32 kernel at orders 3-6, FP64 at orders 4-5), else 0
+    lts = make_lts_setup(order, fractions=(0.6, 0.4), dims=(6, 4, 4), seed=2)
+    eng = Engine(order, arith)
+    run = LocalTimeStepping(eng, DeviceLtsLayer(eng, lts))
+    assert all(t["neigh"] == neigh for t in run.tiles().values()), run.tiles()
+
+
 def test_lts_rejects_missing_setup(gpu):
     lts = make_lts_setup(3, fractions=(0.5, 0.5), dims=(6, 4, 4), seed=1)
     eng64 = Engine(3, "fp64")
